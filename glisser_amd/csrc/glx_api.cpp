@@ -1,0 +1,515 @@
+// C ABI of the gfx950 particle integrator: device memory, streams, events
+// and the host<->device copies around the time-block kernel.
+//
+// Ownership mirrors the reference's split (DeviceData + WHCudaIntegrator +
+// Executor, src/data.cuh:55-67, src/wh.cu:272-287, src/executor.cu:151-175)
+// but in one context: particle SoA arrays, a scratch set for the stable
+// partition, an optional rollback copy, and two planet-block buffers so the
+// upload of block k+1 overlaps the kernel of block k (the planet_data_id
+// parity of src/executor.cu:442-461).  Ordering between the upload and the
+// main stream uses events instead of host synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "errors.h"
+#include "glisser_amd.h"
+#include "launch.h"
+
+using glx::BlockArgs;
+using glx::ParticleSet;
+using glx::PlanetConsts;
+using glx::RollbackSet;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define GLX_HIP(call)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(GLX_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                            \
+    } while (0)
+
+struct PlanetBuf {
+    double* dev_log = nullptr;  // [tbsize_max][n_bodies_max][3]
+    double* pin_log = nullptr;  // pinned staging of the same size
+    hipEvent_t uploaded = nullptr;
+    hipEvent_t consumed = nullptr;
+    uint32_t n_bodies = 0;
+    uint32_t n_steps = 0;
+    double mu = 0;
+    PlanetConsts pc{};
+    bool valid = false;
+};
+
+}  // namespace
+
+namespace glx {
+int set_error(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+void clear_error() { g_err.clear(); }
+}  // namespace glx
+
+struct glx_ctx {
+    int device = 0;
+    uint64_t cap = 0;
+    uint32_t nb_max = 0, tb_max = 0;
+    bool keep_rb = false;
+    hipStream_t main_s = nullptr, htd_s = nullptr;
+    ParticleSet cur{}, tmp{};
+    RollbackSet rb{}, tmp_rb{};
+    uint32_t* counts = nullptr;
+    uint32_t* offsets = nullptr;
+    uint32_t* totals = nullptr;
+    unsigned long long* sum_dev = nullptr;
+    double* dev_aos = nullptr;       // device staging, cap x 3 doubles
+    uint32_t* pin_small = nullptr;   // pinned: totals (2) + sum (2 words)
+    PlanetBuf pb[2];
+    hipEvent_t k0 = nullptr, k1 = nullptr, q0 = nullptr, q1 = nullptr;
+    bool kernel_timed = false, part_timed = false;
+    uint64_t n_alive = 0, n_total = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(glx_ctx* c, T** p, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) + 64);
+    if (e != hipSuccess) return fail(GLX_ENOMEM, "hipMalloc(%zu) failed: %s", count * sizeof(T), hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return GLX_OK;
+}
+
+int alloc_set(glx_ctx* c, ParticleSet* s) {
+    for (int k = 0; k < 9; k++)
+        if (int rc = dalloc(c, &s->d[k], c->cap)) return rc;
+    if (int rc = dalloc(c, &s->flags, c->cap)) return rc;
+    if (int rc = dalloc(c, &s->dti, c->cap)) return rc;
+    return dalloc(c, &s->id, c->cap);
+}
+
+int alloc_rb(glx_ctx* c, RollbackSet* s) {
+    for (int k = 0; k < 6; k++)
+        if (int rc = dalloc(c, &s->d[k], c->cap)) return rc;
+    if (int rc = dalloc(c, &s->flags, c->cap)) return rc;
+    if (int rc = dalloc(c, &s->dti, c->cap)) return rc;
+    return dalloc(c, &s->id, c->cap);
+}
+
+void release(glx_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->main_s) (void)hipStreamSynchronize(c->main_s);
+    if (c->htd_s) (void)hipStreamSynchronize(c->htd_s);
+    for (void* p : c->allocs) (void)hipFree(p);
+    for (auto& b : c->pb) {
+        if (b.pin_log) (void)hipHostFree(b.pin_log);
+        if (b.uploaded) (void)hipEventDestroy(b.uploaded);
+        if (b.consumed) (void)hipEventDestroy(b.consumed);
+    }
+    if (c->pin_small) (void)hipHostFree(c->pin_small);
+    for (hipEvent_t e : {c->k0, c->k1, c->q0, c->q1})
+        if (e) (void)hipEventDestroy(e);
+    if (c->main_s) (void)hipStreamDestroy(c->main_s);
+    if (c->htd_s) (void)hipStreamDestroy(c->htd_s);
+    delete c;
+}
+
+int check_range(const glx_ctx* c, uint64_t begin, uint64_t n) {
+    if (begin > c->cap || n > c->cap - begin)
+        return fail(GLX_EINVAL, "particle range [%llu, %llu) exceeds capacity %llu", (unsigned long long)begin,
+                    (unsigned long long)(begin + n), (unsigned long long)c->cap);
+    return GLX_OK;
+}
+
+// Host AoS vec3 array -> three device SoA arrays at offset `begin`.
+int put_vec3(glx_ctx* c, const double* host, double* const* soa, uint64_t begin, uint64_t n) {
+    GLX_HIP(hipMemcpyAsync(c->dev_aos, host, n * 3 * sizeof(double), hipMemcpyHostToDevice, c->main_s));
+    GLX_HIP(glx::launch_aos_to_soa(c->dev_aos, soa[0] + begin, soa[1] + begin, soa[2] + begin, (uint32_t)n,
+                                   c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    return GLX_OK;
+}
+
+int get_vec3(glx_ctx* c, double* host, double* const* soa, uint64_t begin, uint64_t n) {
+    GLX_HIP(glx::launch_soa_to_aos(soa[0] + begin, soa[1] + begin, soa[2] + begin, c->dev_aos, (uint32_t)n,
+                                   c->main_s));
+    GLX_HIP(hipMemcpyAsync(host, c->dev_aos, n * 3 * sizeof(double), hipMemcpyDeviceToHost, c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    return GLX_OK;
+}
+
+template <typename T>
+int put_plain(glx_ctx* c, const T* host, T* dev, uint64_t n) {
+    GLX_HIP(hipMemcpyAsync(dev, host, n * sizeof(T), hipMemcpyHostToDevice, c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    return GLX_OK;
+}
+
+template <typename T>
+int get_plain(glx_ctx* c, T* host, const T* dev, uint64_t n) {
+    GLX_HIP(hipMemcpyAsync(host, dev, n * sizeof(T), hipMemcpyDeviceToHost, c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    return GLX_OK;
+}
+
+int fill_consts(uint32_t n_bodies, const double* m, const uint32_t* id, const double* rh, PlanetConsts* pc) {
+    std::memset(pc, 0, sizeof *pc);
+    for (uint32_t j = 0; j < n_bodies; j++) {
+        pc->m[j] = m[j];
+        pc->rh2[j] = rh[j] * rh[j];
+        pc->id[j] = id ? id[j] : j;
+    }
+    return GLX_OK;
+}
+
+BlockArgs base_args(glx_ctx* c) {
+    BlockArgs A{};
+    A.rx = c->cur.d[0];
+    A.ry = c->cur.d[1];
+    A.rz = c->cur.d[2];
+    A.vx = c->cur.d[3];
+    A.vy = c->cur.d[4];
+    A.vz = c->cur.d[5];
+    A.ax = c->cur.d[6];
+    A.ay = c->cur.d[7];
+    A.az = c->cur.d[8];
+    A.flags = c->cur.flags;
+    A.dti = c->cur.dti;
+    return A;
+}
+
+constexpr size_t kLdsBudget = 32768;  // bytes of planet rows staged per workgroup
+
+}  // namespace
+
+extern "C" {
+
+const char* glx_last_error(void) { return g_err.c_str(); }
+
+int glx_create(const glx_params* p, glx_ctx** out) {
+    if (!p || !out) return fail(GLX_EINVAL, "glx_create: null argument");
+    *out = nullptr;
+    if (p->n_bodies_max < 1 || p->n_bodies_max > GLX_MAX_BODIES)
+        return fail(GLX_EINVAL, "n_bodies_max %u outside [1, %d]", p->n_bodies_max, GLX_MAX_BODIES);
+    if (p->tbsize_max < 1) return fail(GLX_EINVAL, "tbsize_max must be >= 1");
+    if (p->n_particles > 0xFFFFFFFFull - 2048)
+        return fail(GLX_EINVAL, "n_particles %llu too large", (unsigned long long)p->n_particles);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(GLX_EHIP, "no HIP device available");
+    if (p->device < 0 || p->device >= ndev) return fail(GLX_EINVAL, "device %d out of range", p->device);
+
+    glx_ctx* c = new glx_ctx();
+    c->device = p->device;
+    c->cap = p->n_particles == 0 ? 1 : p->n_particles;
+    c->nb_max = p->n_bodies_max;
+    c->tb_max = p->tbsize_max;
+    c->keep_rb = p->keep_rollback != 0;
+    int rc = GLX_OK;
+    auto bail = [&](int code) {
+        std::string msg = g_err;
+        release(c);
+        g_err = msg;
+        return code;
+    };
+    if (hipSetDevice(c->device) != hipSuccess) return bail(fail(GLX_EHIP, "hipSetDevice failed"));
+    if (hipStreamCreate(&c->main_s) != hipSuccess || hipStreamCreate(&c->htd_s) != hipSuccess)
+        return bail(fail(GLX_EHIP, "hipStreamCreate failed"));
+    if ((rc = alloc_set(c, &c->cur)) || (rc = alloc_set(c, &c->tmp))) return bail(rc);
+    if (c->keep_rb && ((rc = alloc_rb(c, &c->rb)) || (rc = alloc_rb(c, &c->tmp_rb)))) return bail(rc);
+    const size_t tiles = glx::part_tiles(c->cap);
+    if ((rc = dalloc(c, &c->counts, 2 * tiles)) || (rc = dalloc(c, &c->offsets, 2 * tiles)) ||
+        (rc = dalloc(c, &c->totals, 2)) || (rc = dalloc(c, &c->sum_dev, 1)) ||
+        (rc = dalloc(c, &c->dev_aos, 3 * (c->cap > GLX_MAX_BODIES ? c->cap : GLX_MAX_BODIES))))
+        return bail(rc);
+    if (hipHostMalloc((void**)&c->pin_small, 64, hipHostMallocDefault) != hipSuccess)
+        return bail(fail(GLX_ENOMEM, "hipHostMalloc failed"));
+    const size_t log_doubles = (size_t)c->tb_max * c->nb_max * 3;
+    for (auto& b : c->pb) {
+        if ((rc = dalloc(c, &b.dev_log, log_doubles))) return bail(rc);
+        if (hipHostMalloc((void**)&b.pin_log, log_doubles * sizeof(double), hipHostMallocDefault) != hipSuccess)
+            return bail(fail(GLX_ENOMEM, "hipHostMalloc(planet staging) failed"));
+        if (hipEventCreateWithFlags(&b.uploaded, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b.consumed, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(GLX_EHIP, "hipEventCreate failed"));
+        (void)hipEventRecord(b.uploaded, c->htd_s);
+        (void)hipEventRecord(b.consumed, c->main_s);
+    }
+    for (hipEvent_t* ev : {&c->k0, &c->k1, &c->q0, &c->q1})
+        if (hipEventCreate(ev) != hipSuccess) return bail(fail(GLX_EHIP, "hipEventCreate failed"));
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(GLX_EHIP, "hipDeviceSynchronize failed"));
+    *out = c;
+    g_err.clear();
+    return GLX_OK;
+}
+
+int glx_destroy(glx_ctx* c) {
+    release(c);
+    return GLX_OK;
+}
+
+int glx_upload_particles(glx_ctx* c, uint64_t begin, uint64_t n, const double* r, const double* v, const double* a,
+                         const uint16_t* flags, const uint32_t* id, const uint32_t* dti) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_range(c, begin, n)) return rc;
+    if (n == 0) return GLX_OK;
+    if (!r || !v || !flags || !id) return fail(GLX_EINVAL, "glx_upload_particles: r, v, flags, id are required");
+    GLX_HIP(hipSetDevice(c->device));
+    int rc;
+    if ((rc = put_vec3(c, r, c->cur.d, begin, n)) || (rc = put_vec3(c, v, c->cur.d + 3, begin, n))) return rc;
+    if (a && (rc = put_vec3(c, a, c->cur.d + 6, begin, n))) return rc;
+    if ((rc = put_plain(c, flags, c->cur.flags + begin, n)) || (rc = put_plain(c, id, c->cur.id + begin, n)))
+        return rc;
+    if (dti && (rc = put_plain(c, dti, c->cur.dti + begin, n))) return rc;
+    return GLX_OK;
+}
+
+int glx_download_particles(glx_ctx* c, uint64_t begin, uint64_t n, double* r, double* v, double* a,
+                           uint16_t* flags, uint32_t* id, uint32_t* dti) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_range(c, begin, n)) return rc;
+    if (n == 0) return GLX_OK;
+    GLX_HIP(hipSetDevice(c->device));
+    int rc;
+    if (r && (rc = get_vec3(c, r, c->cur.d, begin, n))) return rc;
+    if (v && (rc = get_vec3(c, v, c->cur.d + 3, begin, n))) return rc;
+    if (a && (rc = get_vec3(c, a, c->cur.d + 6, begin, n))) return rc;
+    if (flags && (rc = get_plain(c, flags, c->cur.flags + begin, n))) return rc;
+    if (id && (rc = get_plain(c, id, c->cur.id + begin, n))) return rc;
+    if (dti && (rc = get_plain(c, dti, c->cur.dti + begin, n))) return rc;
+    return GLX_OK;
+}
+
+int glx_set_counts(glx_ctx* c, uint64_t n_alive, uint64_t n_total) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (n_total > c->cap || n_alive > n_total)
+        return fail(GLX_EINVAL, "counts n_alive=%llu n_total=%llu invalid for capacity %llu",
+                    (unsigned long long)n_alive, (unsigned long long)n_total, (unsigned long long)c->cap);
+    c->n_alive = n_alive;
+    c->n_total = n_total;
+    return GLX_OK;
+}
+
+int glx_get_counts(glx_ctx* c, uint64_t* n_alive, uint64_t* n_total) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (n_alive) *n_alive = c->n_alive;
+    if (n_total) *n_total = c->n_total;
+    return GLX_OK;
+}
+
+int glx_upload_planet_block(glx_ctx* c, int buf, uint32_t n_bodies, const double* m, const uint32_t* id,
+                            const double* rh, const double* r_log, const double* h0_log, uint32_t n_steps) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (buf != 0 && buf != 1) return fail(GLX_EINVAL, "planet buffer %d is not 0 or 1", buf);
+    if (n_bodies < 1 || n_bodies > c->nb_max)
+        return fail(GLX_EINVAL, "n_bodies %u outside [1, %u]", n_bodies, c->nb_max);
+    if (n_steps < 1 || n_steps > c->tb_max) return fail(GLX_EINVAL, "n_steps %u outside [1, %u]", n_steps, c->tb_max);
+    if (!m || !rh || !h0_log || (n_bodies > 1 && !r_log)) return fail(GLX_EINVAL, "null planet array");
+    if (!(m[0] > 0.0)) return fail(GLX_EINVAL, "central mass m[0] must be positive");
+    GLX_HIP(hipSetDevice(c->device));
+    PlanetBuf& b = c->pb[buf];
+    // The pinned staging may still feed this buffer's previous copy.
+    GLX_HIP(hipEventSynchronize(b.uploaded));
+    const uint32_t row = n_bodies * 3, prow = (n_bodies - 1) * 3;
+    for (uint32_t s = 0; s < n_steps; s++) {
+        double* dst = b.pin_log + (size_t)s * row;
+        dst[0] = h0_log[3 * (size_t)s];
+        dst[1] = h0_log[3 * (size_t)s + 1];
+        dst[2] = h0_log[3 * (size_t)s + 2];
+        if (prow) std::memcpy(dst + 3, r_log + (size_t)s * prow, prow * sizeof(double));
+    }
+    // Do not overwrite the device buffer while a kernel still reads it.
+    GLX_HIP(hipStreamWaitEvent(c->htd_s, b.consumed, 0));
+    GLX_HIP(hipMemcpyAsync(b.dev_log, b.pin_log, (size_t)n_steps * row * sizeof(double), hipMemcpyHostToDevice,
+                           c->htd_s));
+    GLX_HIP(hipEventRecord(b.uploaded, c->htd_s));
+    fill_consts(n_bodies, m, id, rh, &b.pc);
+    b.mu = m[0];
+    b.n_bodies = n_bodies;
+    b.n_steps = n_steps;
+    b.valid = true;
+    return GLX_OK;
+}
+
+int glx_run_block(glx_ctx* c, int buf, double dt, double outer_radius, int form) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (buf != 0 && buf != 1) return fail(GLX_EINVAL, "planet buffer %d is not 0 or 1", buf);
+    if (form != GLX_FORM_FP64) return fail(GLX_EINVAL, "kernel form %d is not available", form);
+    PlanetBuf& b = c->pb[buf];
+    if (!b.valid) return fail(GLX_ESTATE, "planet buffer %d holds no block", buf);
+    if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
+    GLX_HIP(hipSetDevice(c->device));
+    BlockArgs A = base_args(c);
+    A.log = b.dev_log;
+    A.n = (uint32_t)c->n_alive;
+    A.n_steps = b.n_steps;
+    A.n_bodies = b.n_bodies;
+    const size_t row_bytes = (size_t)b.n_bodies * 3 * sizeof(double);
+    uint32_t chunk = (uint32_t)(kLdsBudget / row_bytes);
+    if (chunk < 1) chunk = 1;
+    if (chunk > b.n_steps) chunk = b.n_steps;
+    A.chunk = chunk;
+    A.dt = dt;
+    A.half_dt = dt / 2;
+    A.mu = b.mu;
+    A.outer2 = outer_radius * outer_radius;
+    A.pc = b.pc;
+    GLX_HIP(hipStreamWaitEvent(c->main_s, b.uploaded, 0));
+    GLX_HIP(hipEventRecord(c->k0, c->main_s));
+    GLX_HIP(glx::launch_block(A, form, c->main_s));
+    GLX_HIP(hipEventRecord(c->k1, c->main_s));
+    GLX_HIP(hipEventRecord(b.consumed, c->main_s));
+    c->kernel_timed = true;
+    return GLX_OK;
+}
+
+int glx_init_accel(glx_ctx* c, uint32_t n_bodies, const double* m, const uint32_t* id, const double* rh,
+                   const double* planet_r, const double* h0, double outer_radius) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (n_bodies < 1 || n_bodies > c->nb_max)
+        return fail(GLX_EINVAL, "n_bodies %u outside [1, %u]", n_bodies, c->nb_max);
+    if (!m || !rh || !h0 || (n_bodies > 1 && !planet_r)) return fail(GLX_EINVAL, "null planet array");
+    GLX_HIP(hipSetDevice(c->device));
+    // Stage one planet row through planet buffer 0's staging space.
+    PlanetBuf& b = c->pb[0];
+    GLX_HIP(hipEventSynchronize(b.uploaded));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    b.pin_log[0] = h0[0];
+    b.pin_log[1] = h0[1];
+    b.pin_log[2] = h0[2];
+    if (n_bodies > 1) std::memcpy(b.pin_log + 3, planet_r, (size_t)(n_bodies - 1) * 3 * sizeof(double));
+    double* row_dev = c->dev_aos;  // free between copies
+    GLX_HIP(hipMemcpyAsync(row_dev, b.pin_log, (size_t)n_bodies * 3 * sizeof(double), hipMemcpyHostToDevice,
+                           c->main_s));
+    BlockArgs A = base_args(c);
+    A.log = row_dev;
+    A.n = (uint32_t)c->n_alive;
+    A.n_steps = 1;
+    A.n_bodies = n_bodies;
+    A.outer2 = outer_radius * outer_radius;
+    fill_consts(n_bodies, m, id, rh, &A.pc);
+    GLX_HIP(glx::launch_init_accel(A, c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    return GLX_OK;
+}
+
+int glx_kill_encounters(glx_ctx* c) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    GLX_HIP(hipSetDevice(c->device));
+    GLX_HIP(glx::launch_kill_encounters(c->cur.flags, (uint32_t)c->n_alive, c->main_s));
+    return GLX_OK;
+}
+
+int glx_partition(glx_ctx* c, int mode, uint64_t* n_unflagged, uint64_t* n_encounter) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (mode != GLX_PART_UNFLAGGED && mode != GLX_PART_THREE_WAY)
+        return fail(GLX_EINVAL, "partition mode %d unknown", mode);
+    GLX_HIP(hipSetDevice(c->device));
+    const uint32_t n = (uint32_t)c->n_alive;
+    uint64_t n0 = 0, n1 = 0;
+    GLX_HIP(hipEventRecord(c->q0, c->main_s));
+    if (n > 0) {
+        GLX_HIP(glx::launch_partition(c->cur, c->tmp, c->keep_rb ? &c->rb : nullptr,
+                                      c->keep_rb ? &c->tmp_rb : nullptr, n, mode == GLX_PART_THREE_WAY, c->counts,
+                                      c->offsets, c->totals, c->main_s));
+        GLX_HIP(hipMemcpyAsync(c->pin_small, c->totals, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->main_s));
+    }
+    GLX_HIP(hipEventRecord(c->q1, c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    if (n > 0) {
+        n0 = c->pin_small[0];
+        n1 = mode == GLX_PART_THREE_WAY ? c->pin_small[1] : 0;
+    }
+    c->n_alive = n0;
+    c->part_timed = true;
+    if (n_unflagged) *n_unflagged = n0;
+    if (n_encounter) *n_encounter = n1;
+    return GLX_OK;
+}
+
+int glx_snapshot_rollback(glx_ctx* c) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (!c->keep_rb) return fail(GLX_ESTATE, "context was created without a rollback copy");
+    GLX_HIP(hipSetDevice(c->device));
+    GLX_HIP(glx::launch_snapshot(c->cur, c->rb, (uint32_t)c->n_total, c->main_s));
+    return GLX_OK;
+}
+
+int glx_download_rollback(glx_ctx* c, uint64_t begin, uint64_t n, double* r, double* v) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (!c->keep_rb) return fail(GLX_ESTATE, "context was created without a rollback copy");
+    if (int rc = check_range(c, begin, n)) return rc;
+    if (n == 0) return GLX_OK;
+    GLX_HIP(hipSetDevice(c->device));
+    int rc;
+    if (r && (rc = get_vec3(c, r, c->rb.d, begin, n))) return rc;
+    if (v && (rc = get_vec3(c, v, c->rb.d + 3, begin, n))) return rc;
+    return GLX_OK;
+}
+
+int glx_sync(glx_ctx* c) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    GLX_HIP(hipSetDevice(c->device));
+    GLX_HIP(hipStreamSynchronize(c->htd_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    return GLX_OK;
+}
+
+int glx_last_times(glx_ctx* c, float* kernel_ms, float* partition_ms) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    GLX_HIP(hipSetDevice(c->device));
+    if (kernel_ms) {
+        *kernel_ms = 0.f;
+        if (c->kernel_timed) {
+            GLX_HIP(hipEventSynchronize(c->k1));
+            GLX_HIP(hipEventElapsedTime(kernel_ms, c->k0, c->k1));
+        }
+    }
+    if (partition_ms) {
+        *partition_ms = 0.f;
+        if (c->part_timed) {
+            GLX_HIP(hipEventSynchronize(c->q1));
+            GLX_HIP(hipEventElapsedTime(partition_ms, c->q0, c->q1));
+        }
+    }
+    return GLX_OK;
+}
+
+int glx_sum_steps(glx_ctx* c, uint64_t n, uint64_t* total) {
+    if (!c || !total) return fail(GLX_EINVAL, "null argument");
+    if (int rc = check_range(c, 0, n)) return rc;
+    GLX_HIP(hipSetDevice(c->device));
+    GLX_HIP(glx::launch_sum_steps(c->cur.dti, (uint32_t)n, c->sum_dev, c->main_s));
+    GLX_HIP(hipMemcpyAsync(c->pin_small + 4, c->sum_dev, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    unsigned long long v;
+    std::memcpy(&v, c->pin_small + 4, sizeof v);
+    *total = v;
+    return GLX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// Device-side pieces of the test-particle Wisdom-Holman step for gfx950.
+//
+// One lane owns one particle for a whole time-block; its heliocentric state
+// (r, v and the acceleration carried between steps) stays in VGPRs.  The
+// per-step planet row (indirect term h0 followed by the heliocentric
+// positions of bodies 1..NB-1) is read from LDS with wave-uniform addresses,
+// so every read is a broadcast.
+//
+// Reference semantics (src/wh.cu:42-177): kick-drift-kick; the drift is the
+// elliptic eccentric-anomaly-difference Kepler solve; the kick sums the
+// planets and the indirect term; encounter (first planet, planet id in the
+// high byte), Sun-cull and outer-limit flags are tested every step and the
+// flagging step completes its second half kick.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "glisser_amd.h"
+
+namespace glx {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTolKep = 1e-14;  // src/wh.cu:13
+constexpr int kKepIts = 5;         // src/wh.cu:12
+
+// Flag bits (src/data.h:169-180).
+constexpr uint32_t kFlagEncounter = 0x01;
+constexpr uint32_t kFlagOut = 0x02;
+constexpr uint32_t kFlagKepler = 0x04;
+constexpr uint32_t kFlagUnbound = 0x08;
+
+struct PlanetConsts {
+    double m[GLX_MAX_BODIES];    // GM of each body, m[0] = Sun
+    double rh2[GLX_MAX_BODIES];  // squared encounter radius (rh[0]: Sun cull radius)
+    uint32_t id[GLX_MAX_BODIES]; // planet id written into the flag high byte
+};
+
+// Arguments of one time-block launch.  Particle arrays are SoA; `log` is
+// the interleaved planet block [n_steps][n_bodies][3]: h0 first, then the
+// heliocentric positions of bodies 1..n_bodies-1.
+struct BlockArgs {
+    double* rx;
+    double* ry;
+    double* rz;
+    double* vx;
+    double* vy;
+    double* vz;
+    double* ax;
+    double* ay;
+    double* az;
+    uint16_t* flags;
+    uint32_t* dti;
+    const double* log;
+    uint32_t n;
+    uint32_t n_steps;
+    uint32_t chunk;     // steps staged in LDS at a time
+    uint32_t n_bodies;  // used by the generic (NB == 0) instantiation
+    double dt;
+    double half_dt;
+    double mu;
+    double outer2;
+    PlanetConsts pc;
+};
+
+struct Particle {
+    double rx, ry, rz, vx, vy, vz, ax, ay, az;
+    uint32_t flags;
+};
+
+// Kepler drift in the reference's arithmetic (src/wh.cu:74-110 with kepeq at
+// src/wh.cu:42-72): fixed five Newton iterations, IEEE divides and square
+// roots, sin/cos from the device math library, and no FMA contraction, so
+// every basic operation rounds exactly as in the reference CPU build (the
+// particle orbits amplify a one-ulp difference to ~1e-10 over 1e5 steps).
+// An unbound orbit (energy >= 0) sets 0x08 and leaves r, v unchanged; the
+// reference GPU code computes a NaN orbit there instead, and its CPU twin
+// throws.
+__device__ __forceinline__ void drift_fp64(Particle& p, double dt, double mu) {
+#pragma clang fp contract(off)
+    const double dist = sqrt(p.rx * p.rx + p.ry * p.ry + p.rz * p.rz);
+    const double vdotr = p.vx * p.rx + p.vy * p.ry + p.vz * p.rz;
+    const double energy = (p.vx * p.vx + p.vy * p.vy + p.vz * p.vz) * 0.5 - mu / dist;
+    if (energy >= 0.0) {
+        p.flags = kFlagUnbound;
+        return;
+    }
+    const double a = -0.5 * mu / energy;
+    const double n_ = sqrt(mu / (a * a * a));
+    const double ecosEo = 1.0 - dist / a;
+    const double esinEo = vdotr / (n_ * a * a);
+    // dt*n - (pi*2)*(int)((dt*n/pi)*2): the reference's M_2PI macro expansion
+    const double dtn = dt * n_;
+    const double dM = dtn - kPi * 2 * (double)(int)(dtn / kPi * 2);
+    const double adv_dt = dM / n_;
+    double sM, cM;
+    sincos(dM, &sM, &cM);
+    double dE = dM - esinEo + esinEo * cM + ecosEo * sM;
+    double sindE, cosdE;
+    sincos(dE, &sindE, &cosdE);
+    double delta = 0.0;
+#pragma unroll
+    for (int k = 0; k < kKepIts; k++) {
+        const double f = dE - ecosEo * sindE + esinEo * (1. - cosdE) - dM;
+        const double fp = 1. - ecosEo * cosdE + esinEo * sindE;
+        delta = -f / fp;
+        dE += delta;
+        sincos(dE, &sindE, &cosdE);
+    }
+    p.flags = fabs(delta) > kTolKep ? kFlagKepler : 0u;
+    const double fp = 1.0 - ecosEo * cosdE + esinEo * sindE;
+    const double f = 1.0 + a * (cosdE - 1.0) / dist;
+    const double g = adv_dt + (sindE - dE) / n_;
+    const double fdot = -n_ * sindE * a / (dist * fp);
+    const double gdot = 1.0 + (cosdE - 1.0) / fp;
+    const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
+    p.rx = rx0 * f + p.vx * g;
+    p.ry = ry0 * f + p.vy * g;
+    p.rz = rz0 * f + p.vz * g;
+    p.vx = rx0 * fdot + p.vx * gdot;
+    p.vy = ry0 * fdot + p.vy * gdot;
+    p.vz = rz0 * fdot + p.vz * gdot;
+}
+
+// Half kick v += a * (dt/2), rounded as the reference CPU build (no FMA).
+__device__ __forceinline__ void kick(Particle& p, double half_dt) {
+#pragma clang fp contract(off)
+    p.vx = p.vx + p.ax * half_dt;
+    p.vy = p.vy + p.ay * half_dt;
+    p.vz = p.vz + p.az * half_dt;
+}
+
+// Acceleration from the step's planet row plus the encounter / cull /
+// outer-limit tests (src/wh.cu:142-170), in FP64 with IEEE divide/sqrt.
+template <int NB>
+__device__ __forceinline__ void accel_fp64(Particle& p, const double* __restrict__ row, const BlockArgs& A,
+                                           uint32_t nb) {
+#pragma clang fp contract(off)
+    p.ax = row[0];
+    p.ay = row[1];
+    p.az = row[2];
+    const uint32_t nbodies = NB > 0 ? (uint32_t)NB : nb;
+#pragma unroll
+    for (uint32_t j = 1; j < (NB > 0 ? (uint32_t)NB : (uint32_t)GLX_MAX_BODIES); j++) {
+        if (NB == 0 && j >= nbodies) break;
+        const double dx = p.rx - row[3 * j];
+        const double dy = p.ry - row[3 * j + 1];
+        const double dz = p.rz - row[3 * j + 2];
+        const double rad = dx * dx + dy * dy + dz * dz;
+        if (rad < A.pc.rh2[j] && p.flags == 0) p.flags = ((A.pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
+        const double inv3 = 1. / (rad * sqrt(rad));
+        const double fac = A.pc.m[j] * inv3;
+        p.ax -= dx * fac;
+        p.ay -= dy * fac;
+        p.az -= dz * fac;
+    }
+    const double rad = p.rx * p.rx + p.ry * p.ry + p.rz * p.rz;
+    if (rad < A.pc.rh2[0]) p.flags = kFlagEncounter;
+    if (rad > A.outer2) p.flags = kFlagOut;
+}
+
+}  // namespace glx

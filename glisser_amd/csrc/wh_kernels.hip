@@ -1,0 +1,141 @@
+// Time-block kernels: every active particle advances through all steps of
+// one planet block (replaces MVSKernel / MVSKernel_, src/wh.cu:15-267).
+//
+// Launch shape: 1-D, 256 lanes per workgroup, one particle per lane.  The
+// planet block is staged into LDS `chunk` steps at a time (the full block
+// at Time-Block-Size 2048 with 5 bodies is 240 KB, above the 160 KB of LDS
+// per CU).  Every lane of a wave reads the same LDS address (broadcast).
+#include <hip/hip_runtime.h>
+
+#include "wh_device.h"
+#include "launch.h"
+
+namespace glx {
+
+template <int NB, int FORM>
+__global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
+    extern __shared__ double lds[];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < A.n;
+    const uint32_t nb = NB > 0 ? (uint32_t)NB : A.n_bodies;
+    const uint32_t row = nb * 3;
+
+    Particle p;
+    uint32_t dti = 0;
+    if (valid) {
+        p.rx = A.rx[i];
+        p.ry = A.ry[i];
+        p.rz = A.rz[i];
+        p.vx = A.vx[i];
+        p.vy = A.vy[i];
+        p.vz = A.vz[i];
+        p.ax = A.ax[i];
+        p.ay = A.ay[i];
+        p.az = A.az[i];
+        p.flags = A.flags[i];
+    } else {
+        p.rx = p.ry = p.rz = p.vx = p.vy = p.vz = p.ax = p.ay = p.az = 0.0;
+        p.flags = 0xFFFFu;  // padding lanes never step
+    }
+
+    for (uint32_t cs = 0; cs < A.n_steps; cs += A.chunk) {
+        const uint32_t cn = min(A.chunk, A.n_steps - cs);
+        __syncthreads();  // every wave is done with the previous chunk
+        const double* src = A.log + (size_t)cs * row;
+        const uint32_t total = cn * row;
+        for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
+        __syncthreads();
+        for (uint32_t s = 0; s < cn; s++) {
+            if (p.flags == 0) {
+                const double* rowp = lds + s * row;
+                kick(p, A.half_dt);
+                drift_fp64(p, A.dt, A.mu);
+                accel_fp64<NB>(p, rowp, A, nb);
+                kick(p, A.half_dt);
+                dti = cs + s + 1;
+            }
+        }
+    }
+
+    if (valid) {
+        A.rx[i] = p.rx;
+        A.ry[i] = p.ry;
+        A.rz[i] = p.rz;
+        A.vx[i] = p.vx;
+        A.vy[i] = p.vy;
+        A.vz[i] = p.vz;
+        A.ax[i] = p.ax;
+        A.ay[i] = p.ay;
+        A.az[i] = p.az;
+        A.flags[i] = (uint16_t)p.flags;
+        A.dti[i] = dti;
+    }
+}
+
+// Initial accelerations and flags from one planet row (t0 positions + h0).
+template <int NB>
+__global__ __launch_bounds__(256) void init_accel_kernel(BlockArgs A) {
+    extern __shared__ double lds[];
+    const uint32_t nb = NB > 0 ? (uint32_t)NB : A.n_bodies;
+    for (uint32_t k = threadIdx.x; k < nb * 3; k += blockDim.x) lds[k] = A.log[k];
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    Particle p;
+    p.rx = A.rx[i];
+    p.ry = A.ry[i];
+    p.rz = A.rz[i];
+    p.flags = A.flags[i];
+    accel_fp64<NB>(p, lds, A, nb);
+    A.ax[i] = p.ax;
+    A.ay[i] = p.ay;
+    A.az[i] = p.az;
+    A.flags[i] = (uint16_t)p.flags;
+}
+
+template <int FORM>
+static hipError_t launch_block_form(const BlockArgs& A, hipStream_t stream) {
+    const uint32_t threads = 256;
+    const uint32_t blocks = (A.n + threads - 1) / threads;
+    const size_t lds_bytes = (size_t)A.chunk * A.n_bodies * 3 * sizeof(double);
+    switch (A.n_bodies) {
+#define GLX_CASE(NBV)                                                                          \
+    case NBV:                                                                                  \
+        hipLaunchKernelGGL((wh_block_kernel<NBV, FORM>), dim3(blocks), dim3(threads), lds_bytes, \
+                           stream, A);                                                         \
+        break;
+        GLX_CASE(2)
+        GLX_CASE(3)
+        GLX_CASE(4)
+        GLX_CASE(5)
+        GLX_CASE(6)
+        GLX_CASE(7)
+        GLX_CASE(8)
+        GLX_CASE(9)
+#undef GLX_CASE
+        default:
+            hipLaunchKernelGGL((wh_block_kernel<0, FORM>), dim3(blocks), dim3(threads), lds_bytes, stream, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_block(const BlockArgs& A, int form, hipStream_t stream) {
+    if (A.n == 0) return hipSuccess;
+    switch (form) {
+        case GLX_FORM_FP64:
+            return launch_block_form<GLX_FORM_FP64>(A, stream);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_init_accel(const BlockArgs& A, hipStream_t stream) {
+    if (A.n == 0) return hipSuccess;
+    const uint32_t threads = 256;
+    const uint32_t blocks = (A.n + threads - 1) / threads;
+    const size_t lds_bytes = (size_t)A.n_bodies * 3 * sizeof(double);
+    hipLaunchKernelGGL((init_accel_kernel<0>), dim3(blocks), dim3(threads), lds_bytes, stream, A);
+    return hipGetLastError();
+}
+
+}  // namespace glx

@@ -1,0 +1,318 @@
+"""ctypes binding of ``libglisser_amd.so`` (the C ABI in include/glisser_amd.h).
+
+The library is loaded from the package folder only.  If it is missing or
+cannot be loaded the import fails loudly: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libglisser_amd.so"
+
+GLX_OK, GLX_EINVAL, GLX_EHIP, GLX_ENOMEM, GLX_ESTATE = 0, 1, 2, 3, 4
+FORM_FP64, FORM_FAST, FORM_MIXED = 0, 1, 2
+FORMS = {"fp64": FORM_FP64, "fast": FORM_FAST, "mixed": FORM_MIXED}
+PART_UNFLAGGED, PART_THREE_WAY = 0, 1
+MAX_BODIES = 16
+
+_dp = C.POINTER(C.c_double)
+_u16p = C.POINTER(C.c_uint16)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class GlxError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"glx error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_uint64),
+        ("n_bodies_max", C.c_uint32),
+        ("tbsize_max", C.c_uint32),
+        ("device", C.c_int32),
+        ("keep_rollback", C.c_int32),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/glisser_amd.h
+SIGNATURES = {
+    "glx_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "glx_destroy": (C.c_int, [C.c_void_p]),
+    "glx_last_error": (C.c_char_p, []),
+    "glx_upload_particles": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, _dp, _dp, _dp, _u16p, _u32p, _u32p]),
+    "glx_download_particles": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, _dp, _dp, _dp, _u16p, _u32p, _u32p]),
+    "glx_set_counts": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
+    "glx_get_counts": (C.c_int, [C.c_void_p, _u64p, _u64p]),
+    "glx_upload_planet_block": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, _dp, _u32p, _dp, _dp, _dp, C.c_uint32]),
+    "glx_run_block": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
+    "glx_init_accel": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _u32p, _dp, _dp, _dp, C.c_double]),
+    "glx_kill_encounters": (C.c_int, [C.c_void_p]),
+    "glx_partition": (C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p]),
+    "glx_snapshot_rollback": (C.c_int, [C.c_void_p]),
+    "glx_download_rollback": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, _dp, _dp]),
+    "glx_sync": (C.c_int, [C.c_void_p]),
+    "glx_last_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "glx_sum_steps": (C.c_int, [C.c_void_p, C.c_uint64, _u64p]),
+    "glx_planets_create": (C.c_int, [C.c_uint32, _dp, _dp, _dp, _u32p, C.c_double, C.c_double, C.c_double,
+                                     C.POINTER(C.c_void_p)]),
+    "glx_planets_destroy": (C.c_int, [C.c_void_p]),
+    "glx_planets_h0_initial": (C.c_int, [C.c_void_p, _dp]),
+    "glx_planets_block": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, _dp, _dp, _dp]),
+    "glx_planets_state": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "glx_planets_recalculate_rh": (C.c_int, [C.c_void_p]),
+    "glx_planets_metrics": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "glx_load_h0": (C.c_int, [C.c_uint32, _dp, _dp, C.c_uint32, _dp]),
+    "glx_to_elements": (C.c_int, [C.c_uint64, _dp, _dp, _dp, _dp]),
+    "glx_from_elements": (C.c_int, [C.c_uint64, _dp, _dp, C.c_int, _dp, _dp]),
+}
+
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m glisser_amd.build` "
+                "(the particle integrator has no CPU fallback)")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _ptr(a, typ):
+    if a is None:
+        return None
+    return a.ctypes.data_as(typ)
+
+
+def _vec3(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape != (n, 3):
+        raise ValueError(f"expected shape ({n}, 3), got {a.shape}")
+    return a
+
+
+class Context:
+    """One device context: particle arrays, planet-block buffers, streams."""
+
+    def __init__(self, n_particles: int, n_bodies_max: int, tbsize_max: int, device: int = 0,
+                 keep_rollback: bool = False):
+        self.lib = load()
+        p = Params(n_particles, n_bodies_max, tbsize_max, device, int(keep_rollback))
+        h = C.c_void_p()
+        self._check(self.lib.glx_create(C.byref(p), C.byref(h)))
+        self.h = h
+        self.capacity = n_particles
+
+    def _check(self, rc: int) -> None:
+        if rc != GLX_OK:
+            raise GlxError(rc, self.lib.glx_last_error().decode())
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.glx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- particles -----------------------------------------------------
+    def upload(self, r, v, flags, ids, a=None, dti=None, begin: int = 0) -> None:
+        n = len(flags)
+        r, v = _vec3(r, n), _vec3(v, n)
+        a = None if a is None else _vec3(a, n)
+        flags = np.ascontiguousarray(flags, dtype=np.uint16)
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        dti = None if dti is None else np.ascontiguousarray(dti, dtype=np.uint32)
+        self._check(self.lib.glx_upload_particles(self.h, begin, n, _ptr(r, _dp), _ptr(v, _dp), _ptr(a, _dp),
+                                                  _ptr(flags, _u16p), _ptr(ids, _u32p), _ptr(dti, _u32p)))
+
+    def download(self, n: int, begin: int = 0) -> dict:
+        out = {
+            "r": np.empty((n, 3)), "v": np.empty((n, 3)), "a": np.empty((n, 3)),
+            "flags": np.empty(n, np.uint16), "id": np.empty(n, np.uint32), "dti": np.empty(n, np.uint32),
+        }
+        self._check(self.lib.glx_download_particles(
+            self.h, begin, n, _ptr(out["r"], _dp), _ptr(out["v"], _dp), _ptr(out["a"], _dp),
+            _ptr(out["flags"], _u16p), _ptr(out["id"], _u32p), _ptr(out["dti"], _u32p)))
+        return out
+
+    def set_counts(self, n_alive: int, n_total: int) -> None:
+        self._check(self.lib.glx_set_counts(self.h, n_alive, n_total))
+
+    def counts(self) -> tuple[int, int]:
+        a, t = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.glx_get_counts(self.h, C.byref(a), C.byref(t)))
+        return a.value, t.value
+
+    # --- planets / kernels -----------------------------------------------
+    def upload_planet_block(self, buf: int, m, rh, r_log, h0_log, ids=None) -> None:
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        nb = len(m)
+        rh = np.ascontiguousarray(rh, dtype=np.float64)
+        h0 = np.ascontiguousarray(h0_log, dtype=np.float64).reshape(-1, 3)
+        steps = h0.shape[0]
+        rl = np.ascontiguousarray(r_log, dtype=np.float64).reshape(steps, nb - 1, 3)
+        ids = np.arange(nb, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+        self._check(self.lib.glx_upload_planet_block(self.h, buf, nb, _ptr(m, _dp), _ptr(ids, _u32p),
+                                                     _ptr(rh, _dp), _ptr(rl, _dp), _ptr(h0, _dp), steps))
+
+    def run_block(self, buf: int, dt: float, outer_radius: float, form: str | int = "fp64") -> None:
+        f = FORMS[form] if isinstance(form, str) else int(form)
+        self._check(self.lib.glx_run_block(self.h, buf, dt, outer_radius, f))
+
+    def init_accel(self, m, rh, planet_r, h0, outer_radius: float, ids=None) -> None:
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        nb = len(m)
+        rh = np.ascontiguousarray(rh, dtype=np.float64)
+        pr = np.ascontiguousarray(planet_r, dtype=np.float64).reshape(nb - 1, 3)
+        h0 = np.ascontiguousarray(h0, dtype=np.float64).reshape(3)
+        ids = np.arange(nb, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+        self._check(self.lib.glx_init_accel(self.h, nb, _ptr(m, _dp), _ptr(ids, _u32p), _ptr(rh, _dp),
+                                            _ptr(pr, _dp), _ptr(h0, _dp), outer_radius))
+
+    def kill_encounters(self) -> None:
+        self._check(self.lib.glx_kill_encounters(self.h))
+
+    def partition(self, three_way: bool = False) -> tuple[int, int]:
+        a, e = C.c_uint64(), C.c_uint64()
+        mode = PART_THREE_WAY if three_way else PART_UNFLAGGED
+        self._check(self.lib.glx_partition(self.h, mode, C.byref(a), C.byref(e)))
+        return a.value, e.value
+
+    def snapshot_rollback(self) -> None:
+        self._check(self.lib.glx_snapshot_rollback(self.h))
+
+    def download_rollback(self, n: int, begin: int = 0):
+        r, v = np.empty((n, 3)), np.empty((n, 3))
+        self._check(self.lib.glx_download_rollback(self.h, begin, n, _ptr(r, _dp), _ptr(v, _dp)))
+        return r, v
+
+    def sync(self) -> None:
+        self._check(self.lib.glx_sync(self.h))
+
+    def last_times(self) -> tuple[float, float]:
+        k, p = C.c_float(), C.c_float()
+        self._check(self.lib.glx_last_times(self.h, C.byref(k), C.byref(p)))
+        return k.value, p.value
+
+    def sum_steps(self, n: int) -> int:
+        t = C.c_uint64()
+        self._check(self.lib.glx_sum_steps(self.h, n, C.byref(t)))
+        return t.value
+
+
+def _check_host(rc: int) -> None:
+    if rc != GLX_OK:
+        raise GlxError(rc, load().glx_last_error().decode())
+
+
+class Planets:
+    """Host planet producer (Jacobi WH): one planet log + h0 log per block."""
+
+    def __init__(self, m, r, v, ids=None, cull_radius: float = 0.5, rh_factor: float = 0.0,
+                 outer_radius: float = 300.0):
+        self.lib = load()
+        self.m = np.ascontiguousarray(m, dtype=np.float64)
+        self.n = len(self.m)
+        r = _vec3(r, self.n)
+        v = _vec3(v, self.n)
+        self.ids = (np.arange(self.n, dtype=np.uint32) if ids is None
+                    else np.ascontiguousarray(ids, dtype=np.uint32))
+        self.outer_radius = outer_radius
+        h = C.c_void_p()
+        _check_host(self.lib.glx_planets_create(self.n, _ptr(self.m, _dp), _ptr(r, _dp), _ptr(v, _dp),
+                                                _ptr(self.ids, _u32p), cull_radius, rh_factor, outer_radius,
+                                                C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.glx_planets_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def h0_initial(self) -> np.ndarray:
+        out = np.empty(3)
+        _check_host(self.lib.glx_planets_h0_initial(self.h, _ptr(out, _dp)))
+        return out
+
+    def block(self, n_steps: int, dt: float):
+        """Advance n_steps; returns (r_log, v_log, h0_log)."""
+        rl = np.empty((n_steps, self.n - 1, 3))
+        vl = np.empty((n_steps, self.n - 1, 3))
+        h0 = np.empty((n_steps, 3))
+        _check_host(self.lib.glx_planets_block(self.h, n_steps, dt, _ptr(rl, _dp), _ptr(vl, _dp), _ptr(h0, _dp)))
+        return rl, vl, h0
+
+    def state(self):
+        r, v, rh = np.empty((self.n, 3)), np.empty((self.n, 3)), np.empty(self.n)
+        _check_host(self.lib.glx_planets_state(self.h, _ptr(r, _dp), _ptr(v, _dp), _ptr(rh, _dp)))
+        return r, v, rh
+
+    def recalculate_rh(self) -> None:
+        _check_host(self.lib.glx_planets_recalculate_rh(self.h))
+
+    def metrics(self):
+        e = np.empty(1)
+        lv = np.empty(3)
+        _check_host(self.lib.glx_planets_metrics(self.h, _ptr(e, _dp), _ptr(lv, _dp)))
+        return float(e[0]), lv
+
+
+def load_h0(m, r_log) -> np.ndarray:
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    rl = np.ascontiguousarray(r_log, dtype=np.float64)
+    steps = rl.shape[0]
+    h0 = np.empty((steps, 3))
+    _check_host(load().glx_load_h0(len(m), _ptr(m, _dp), _ptr(rl, _dp), steps, _ptr(h0, _dp)))
+    return h0
+
+
+def to_elements(mu, r, v) -> np.ndarray:
+    r = np.ascontiguousarray(np.atleast_2d(r), dtype=np.float64)
+    v = np.ascontiguousarray(np.atleast_2d(v), dtype=np.float64)
+    n = r.shape[0]
+    mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (n,)))
+    el = np.empty((n, 6))
+    _check_host(load().glx_to_elements(n, _ptr(mu, _dp), _ptr(r, _dp), _ptr(v, _dp), _ptr(el, _dp)))
+    return el
+
+
+def from_elements(mu, el, mean_anomaly: bool = True):
+    el = np.ascontiguousarray(np.atleast_2d(el), dtype=np.float64)
+    n = el.shape[0]
+    mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (n,)))
+    r, v = np.empty((n, 3)), np.empty((n, 3))
+    _check_host(load().glx_from_elements(n, _ptr(mu, _dp), _ptr(el, _dp), int(mean_anomaly), _ptr(r, _dp),
+                                         _ptr(v, _dp)))
+    return r, v

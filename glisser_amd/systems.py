@@ -1,0 +1,93 @@
+"""Seeded synthetic planetary systems and test-particle swarms.
+
+Units: au, day; masses in GM units (Big-G = 1), GM_sun = 2.959122e-4.
+Elements become Cartesian states with the reference's ``from_elements_M``
+convention (src/convert.cpp:230-308): planets about GM_sun (1 + mu_i),
+particles about GM_sun.  The generator is numpy's PCG64 seeded per call;
+the seed is recorded in every returned system.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import native
+
+GM_SUN = 2.959122e-4
+DEG = np.pi / 180.0
+
+# Sun + 4 giants: mass ratios and semi-major axes (au)
+GIANTS_MU = np.array([9.55e-4, 2.86e-4, 4.37e-5, 5.15e-5])
+GIANTS_A = np.array([5.20, 9.55, 19.2, 30.1])
+
+
+@dataclass
+class System:
+    m: np.ndarray          # (nb,) GM of Sun + planets
+    r: np.ndarray          # (nb, 3) heliocentric planets (Sun at 0)
+    v: np.ndarray
+    pr: np.ndarray         # (n, 3) particles
+    pv: np.ndarray
+    ids: np.ndarray        # (n,) particle ids
+    seed: int
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_bodies(self) -> int:
+        return len(self.m)
+
+    @property
+    def n(self) -> int:
+        return len(self.ids)
+
+
+def _elements(rng, n, a_lo, a_hi, e_hi, i_hi_deg):
+    el = np.empty((n, 6))
+    el[:, 0] = rng.uniform(a_lo, a_hi, n)
+    el[:, 1] = rng.uniform(0.0, e_hi, n)
+    el[:, 2] = rng.uniform(0.0, i_hi_deg * DEG, n)
+    el[:, 3:6] = rng.uniform(0.0, 2 * np.pi, (n, 3))  # Omega, omega, M
+    return el
+
+
+def planets(rng, mus=GIANTS_MU, axes=GIANTS_A, e_hi=0.05, i_hi_deg=2.0):
+    """Sun + planets on seeded low-e, low-i orbits; returns (m, r, v)."""
+    npl = len(mus)
+    el = np.empty((npl, 6))
+    el[:, 0] = axes
+    el[:, 1] = rng.uniform(0.0, e_hi, npl)
+    el[:, 2] = rng.uniform(0.0, i_hi_deg * DEG, npl)
+    el[:, 3:6] = rng.uniform(0.0, 2 * np.pi, (npl, 3))
+    m = np.concatenate([[GM_SUN], GM_SUN * np.asarray(mus, dtype=np.float64)])
+    pr, pv = native.from_elements(m[0] + m[1:], el, mean_anomaly=True)
+    r = np.vstack([np.zeros(3), pr])
+    v = np.vstack([np.zeros(3), pv])
+    return m, r, v
+
+
+def outer_solar_system(n_particles: int, seed: int = 1, a_range=(60.0, 65.0), e_hi=0.1,
+                       i_hi_deg=5.0) -> System:
+    """Sun + 4 giants (a = 5.20/9.55/19.2/30.1 au, e < 0.05, i < 2 deg) and
+    a particle swarm with a ~ U[a_range], e ~ U[0, e_hi], i ~ U[0, i_hi]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    m, r, v = planets(rng)
+    el = _elements(rng, n_particles, a_range[0], a_range[1], e_hi, i_hi_deg)
+    pr, pv = native.from_elements(np.full(n_particles, m[0]), el, mean_anomaly=True)
+    return System(m, r, v, pr, pv, np.arange(n_particles, dtype=np.uint32), seed,
+                  {"kind": "outer_solar_system", "a_range": a_range, "e_hi": e_hi, "i_hi_deg": i_hi_deg})
+
+
+def three_body(n_particles: int, seed: int = 4, a_pl=30.0, mu_pl=1e-3, a_range=(25.0, 40.0), e_hi=0.3,
+               i_hi_deg=10.0) -> System:
+    """Sun + one planet on a circular orbit and a swarm around it."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    m = np.array([GM_SUN, GM_SUN * mu_pl])
+    el = np.array([[a_pl, 0.0, 0.0, 0.0, 0.0, 0.0]])
+    pr, pv = native.from_elements(m[0] + m[1:], el, mean_anomaly=True)
+    r = np.vstack([np.zeros(3), pr])
+    v = np.vstack([np.zeros(3), pv])
+    el = _elements(rng, n_particles, a_range[0], a_range[1], e_hi, i_hi_deg)
+    ppr, ppv = native.from_elements(np.full(n_particles, m[0]), el, mean_anomaly=True)
+    return System(m, r, v, ppr, ppv, np.arange(n_particles, dtype=np.uint32), seed,
+                  {"kind": "three_body", "a_pl": a_pl, "mu_pl": mu_pl})

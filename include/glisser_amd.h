@@ -1,0 +1,197 @@
+/*
+ * glisser_amd.h -- C ABI of the MI355X (gfx950) test-particle time-block
+ * integrator.
+ *
+ * The entry points replace the device half of GLISSER's particle path:
+ * the per-particle Wisdom-Holman time-block kernel (src/wh.cu), the
+ * encounter-kill transform and the partition that follow it in the executor
+ * (src/executor.cu), and the host<->device copies that feed them.  Plain
+ * pointers and sizes only; every call returns a status (0 = success) and
+ * glx_last_error() describes the last failure on the calling thread.  No
+ * C++ exception crosses this boundary.
+ *
+ * Host-side layout of vector arrays is the reference's f64_3 layout
+ * (src/types.h:78-213): x, y, z doubles with a 24-byte stride.  On the
+ * device the context keeps particles as structure-of-arrays.
+ *
+ * Threading: one host thread per context.  Calls marked "async" only enqueue
+ * work on the context's streams; glx_sync() waits for all of it.
+ */
+#ifndef GLISSER_AMD_H
+#define GLISSER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glx_ctx glx_ctx;
+
+enum glx_status {
+    GLX_OK = 0,
+    GLX_EINVAL = 1,  /* bad argument (size, index, null pointer) */
+    GLX_EHIP = 2,    /* a HIP runtime call failed */
+    GLX_ENOMEM = 3,  /* device or pinned allocation failed */
+    GLX_ESTATE = 4   /* call not valid in the context's current state */
+};
+
+/* Arithmetic forms of the time-block kernel. */
+enum glx_form {
+    GLX_FORM_FP64 = 0,  /* reference arithmetic: fixed 5 Newton iterations, IEEE div/sqrt */
+    GLX_FORM_FAST = 1,  /* FP64 throughout, fewer and cheaper operations (early-exit Newton) */
+    GLX_FORM_MIXED = 2  /* FAST drift + FP32 r^-3 in the planet kick, FP64 accumulation */
+};
+
+/* Partition modes of glx_partition(). */
+enum glx_partition_mode {
+    GLX_PART_UNFLAGGED = 0, /* flags == 0 first, everything else after (stable) */
+    GLX_PART_THREE_WAY = 1  /* flags == 0, then (flags & 0xFE) == 0, then the rest (stable) */
+};
+
+#define GLX_MAX_BODIES 16
+
+typedef struct glx_params {
+    uint64_t n_particles;  /* capacity; DeviceParticlePhaseSpace(n) at src/executor.cu:167 */
+    uint32_t n_bodies_max; /* Sun + planets, <= GLX_MAX_BODIES */
+    uint32_t tbsize_max;   /* Time-Block-Size: rows of one planet log */
+    int32_t device;        /* HIP device ordinal */
+    int32_t keep_rollback; /* allocate the rollback copy (Resolve-Encounters 1) */
+} glx_params;
+
+/* Create / destroy a context and all its device memory.
+ * Replaces the allocations of Executor::init (src/executor.cu:151-175) and
+ * WHCudaIntegrator's constructor (src/wh.cu:272-282). */
+int glx_create(const glx_params* params, glx_ctx** out);
+int glx_destroy(glx_ctx* ctx);
+
+/* Text of the last error on this thread ("" if none). */
+const char* glx_last_error(void);
+
+/* Copy particles [begin, begin+n) host -> device (synchronous).
+ * r, v, a: n x 3 doubles; flags, id, deathtime_index: n entries.  a and
+ * deathtime_index may be NULL (then left unchanged).
+ * Replaces Executor::upload_data (src/executor.cu:322-337) together with
+ * WHCudaIntegrator::upload_data_cuda (src/wh.cu:338-342). */
+int glx_upload_particles(glx_ctx* ctx, uint64_t begin, uint64_t n, const double* r, const double* v,
+                         const double* a, const uint16_t* flags, const uint32_t* id,
+                         const uint32_t* deathtime_index);
+
+/* Copy particles [begin, begin+n) device -> host (synchronous); any output
+ * pointer may be NULL.  Replaces Executor::download_data
+ * (src/executor.cu:358-440). */
+int glx_download_particles(glx_ctx* ctx, uint64_t begin, uint64_t n, double* r, double* v, double* a,
+                           uint16_t* flags, uint32_t* id, uint32_t* deathtime_index);
+
+/* Number of particles the next glx_run_block advances ([0, n_alive)) and the
+ * total count held (n_total >= n_alive).  DeviceParticlePhaseSpace::n_alive /
+ * n_total (src/data.cuh:8-30). */
+int glx_set_counts(glx_ctx* ctx, uint64_t n_alive, uint64_t n_total);
+int glx_get_counts(glx_ctx* ctx, uint64_t* n_alive, uint64_t* n_total);
+
+/* Upload one planet block into buffer buf (0 or 1), async on the upload
+ * stream.  m, rh: n_bodies doubles (GM units; m[0] is the Sun); id: n_bodies
+ * planet ids; r_log: n_steps x (n_bodies-1) x 3 heliocentric planet
+ * positions, row s at t0 + (s+1) dt; h0_log: n_steps x 3 indirect terms.
+ * The host arrays may be reused as soon as the call returns.
+ * Replaces Executor::upload_planet_log (src/executor.cu:442-461) and
+ * WHCudaIntegrator::upload_planet_log_cuda (src/wh.cu:306-311). */
+int glx_upload_planet_block(glx_ctx* ctx, int buf, uint32_t n_bodies, const double* m, const uint32_t* id,
+                            const double* rh, const double* r_log, const double* h0_log, uint32_t n_steps);
+
+/* Advance particles [0, n_alive) through every step of planet buffer buf
+ * (async on the main stream; waits for that buffer's upload).
+ * Replaces WHCudaIntegrator::integrate_particles_timeblock_cuda
+ * (src/wh.cu:344-395) and the MVSKernel it launches (src/wh.cu:15-207). */
+int glx_run_block(glx_ctx* ctx, int buf, double dt, double outer_radius, int form);
+
+/* Initial accelerations and flags of particles [0, n_alive) from planet row
+ * 0 of `planet_r` ((n_bodies-1) x 3) and h0 (3 doubles), async.  The
+ * device-side counterpart of the helio_acc_particles<true> call in the
+ * WHIntegrator constructor (src/wh.cpp:531-543). */
+int glx_init_accel(glx_ctx* ctx, uint32_t n_bodies, const double* m, const uint32_t* id, const double* rh,
+                   const double* planet_r, const double* h0, double outer_radius);
+
+/* flags & 0x01 -> (flags & 0xFF00) | 0x80 over [0, n_alive), async.
+ * Replaces KillEncounterKernel (src/executor.cu:63-78,656-660). */
+int glx_kill_encounters(glx_ctx* ctx);
+
+/* Stable partition of [0, n_alive) moving every particle array (and the
+ * rollback copy when kept); synchronous.  *n_unflagged receives the new
+ * n_alive; *n_encounter the size of the encounter class (THREE_WAY) or 0.
+ * Replaces the thrust::partition calls of src/executor.cu:663-665,780-797
+ * with an order-preserving (std::stable_partition-equal) compaction. */
+int glx_partition(glx_ctx* ctx, int mode, uint64_t* n_unflagged, uint64_t* n_encounter);
+
+/* Copy r, v, flags, deathtime_index and id of [0, n_total) into the rollback
+ * copy, async.  Replaces the memcpy_dtd block of src/executor.cu:624-635. */
+int glx_snapshot_rollback(glx_ctx* ctx);
+
+/* Download the rollback copy of [begin, begin+n) (r, v may be NULL). */
+int glx_download_rollback(glx_ctx* ctx, uint64_t begin, uint64_t n, double* r, double* v);
+
+/* Wait for all work queued on the context. */
+int glx_sync(glx_ctx* ctx);
+
+/* Device time of the last glx_run_block kernel and of the last
+ * glx_partition, in milliseconds (HIP events on the launching stream);
+ * the cudaEventElapsedTime pairs of src/executor.cu:637-648,776-806. */
+int glx_last_times(glx_ctx* ctx, float* kernel_ms, float* partition_ms);
+
+/* Sum of deathtime_index over [0, n) -- the particle-steps done by the last
+ * block -- computed on the device (synchronous). */
+int glx_sum_steps(glx_ctx* ctx, uint64_t n, uint64_t* total);
+
+/* ---------------------------------------------------------------------
+ * Host-side producer of the kernel's planet input and orbital conversions.
+ * Built without FP contraction so it rounds like the reference CPU build.
+ * ------------------------------------------------------------------- */
+
+typedef struct glx_planets glx_planets;
+
+/* Planet Jacobi Wisdom-Holman integrator: the planet half of the
+ * WHIntegrator constructor (src/wh.cpp:498-531).  m in GM units (m[0] is
+ * the Sun), heliocentric r, v (n x 3); id may be NULL (0..n-1). */
+int glx_planets_create(uint32_t n, const double* m, const double* r, const double* v, const uint32_t* id,
+                       double cull_radius, double rh_factor, double outer_radius, glx_planets** out);
+int glx_planets_destroy(glx_planets* p);
+
+/* Indirect term h0 at the initial time (the constructor's h0_log.old[0],
+ * src/wh.cpp:531-540). */
+int glx_planets_h0_initial(glx_planets* p, double* h0);
+
+/* n_steps planet steps: WHIntegrator::integrate_planets_timeblock
+ * (src/wh.cpp:584-595) with step_planets (src/wh.cpp:1027-1067).
+ * r_log, v_log: n_steps x (n-1) x 3; h0_log: n_steps x 3 (any may be NULL).
+ * GLX_ESTATE on an unbound or unconverged planet orbit (where the
+ * reference throws). */
+int glx_planets_block(glx_planets* p, uint32_t n_steps, double dt, double* r_log, double* v_log, double* h0_log);
+
+/* Current heliocentric state (n x 3) and encounter radii (n); any output may
+ * be NULL. */
+int glx_planets_state(glx_planets* p, double* r, double* v, double* rh);
+
+/* WHIntegrator::recalculate_rh (src/wh.cpp:472-495). */
+int glx_planets_recalculate_rh(glx_planets* p);
+
+/* calculate_planet_metrics (src/wh.cpp:422-470): energy and L (3). */
+int glx_planets_metrics(glx_planets* p, double* energy, double* l);
+
+/* WHIntegrator::load_h0 (src/wh.cpp:993-1025): h0 rows from a position log
+ * of bodies 1..n_bodies-1. */
+int glx_load_h0(uint32_t n_bodies, const double* m, const double* r_log, uint32_t n_steps, double* h0_log);
+
+/* to_elements (src/convert.cpp:310-463) for n states about mu[k];
+ * el: n x 6 (a, e, i, Omega, omega, f). */
+int glx_to_elements(uint64_t n, const double* mu, const double* r, const double* v, double* el);
+
+/* from_elements / from_elements_M (src/convert.cpp:230-308) for n element
+ * sets el (n x 6: a, e, i, Omega, omega, f-or-M). */
+int glx_from_elements(uint64_t n, const double* mu, const double* el, int anomaly_is_mean, double* r, double* v);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLISSER_AMD_H */

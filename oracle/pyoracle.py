@@ -1,0 +1,206 @@
+"""ctypes wrapper of the CPU oracle libraries (TEST INFRASTRUCTURE ONLY).
+
+``Oracle()`` loads ``oracle/liboracle.so`` (the repo's restatement of the
+reference CPU WH path); ``Oracle(reference=True)`` loads
+``oracle/_ref/libglisser_ref.so`` (the reference's own CPU sources, built in
+the container that holds them).  Both expose the same entry points.
+Only tests/, the smoke entry point and bench.py's CPU-baseline leg use it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libglisser_ref.so"
+
+_dp = C.POINTER(C.c_double)
+_u16p = C.POINTER(C.c_uint16)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+
+_SIGS = {
+    "last_error": (C.c_char_p, []),
+    "particle_accels": (None, [C.c_uint32, _dp, _dp, C.c_double, _dp, _dp, C.c_double, C.c_uint32, _dp, _dp,
+                               _u16p, _u32p, _dp]),
+    "particle_block": (C.c_int, [C.c_uint32, _dp, _dp, C.c_double, _dp, _dp, C.c_uint32, C.c_double, C.c_double,
+                                 C.c_uint32, _dp, _dp, _dp, _u16p, _u32p, _u32p, _dp, _u64p]),
+    "planets_new": (C.c_void_p, [C.c_uint32, _dp, _dp, _dp, _u32p, C.c_double, C.c_double, C.c_double, _dp]),
+    "planets_free": (None, [C.c_void_p]),
+    "planets_block": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, _dp, _dp, _dp]),
+    "planets_get": (None, [C.c_void_p, _dp, _dp, _dp]),
+    "planets_recalculate_rh": (None, [C.c_void_p]),
+    "load_h0": (None, [C.c_uint32, _dp, _dp, C.c_uint32, _dp]),
+    "planet_metrics": (None, [C.c_uint32, _dp, _dp, _dp, _dp, _dp]),
+    "stable_partition": (C.c_uint64, [_u16p, C.c_uint64, C.c_uint64, C.c_int, _u64p]),
+    "to_elements": (None, [C.c_double, _dp, _dp, _dp]),
+    "from_elements": (None, [C.c_double, _dp, _dp, _dp]),
+}
+_ORACLE_ONLY = {"drift": (C.c_int, [C.c_double, C.c_uint32, _dp, _dp, _dp, C.c_int, _u64p])}
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def reference_available() -> bool:
+    return REF_LIB.exists()
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"oracle status {code}: {msg}")
+        self.code = code
+
+
+class Oracle:
+    def __init__(self, reference: bool = False):
+        path = REF_LIB if reference else ORACLE_LIB
+        if not path.exists():
+            raise FileNotFoundError(f"{path} not built (make -C oracle {'ref' if reference else 'oracle'})")
+        self.lib = C.CDLL(str(path))
+        self.prefix = "ref_" if reference else "orc_"
+        self.reference = reference
+        sigs = dict(_SIGS)
+        if not reference:
+            sigs.update(_ORACLE_ONLY)
+        for name, (res, args) in sigs.items():
+            fn = getattr(self.lib, self.prefix + name)
+            fn.restype = res
+            fn.argtypes = args
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def err(self) -> str:
+        return self._f("last_error")().decode()
+
+    # -- particles ------------------------------------------------------
+    def particle_accels(self, m, rh, outer, planet_r, h0, r, flags, ids, time=0.0, deathtime=None):
+        m = np.ascontiguousarray(m, np.float64)
+        rh = np.ascontiguousarray(rh, np.float64)
+        pr = np.ascontiguousarray(planet_r, np.float64)
+        h0 = np.ascontiguousarray(h0, np.float64)
+        r = np.ascontiguousarray(r, np.float64)
+        n = r.shape[0]
+        a = np.empty((n, 3))
+        flags = np.array(flags, dtype=np.uint16, copy=True)
+        ids = np.ascontiguousarray(ids, np.uint32)
+        self._f("particle_accels")(len(m), _p(m, _dp), _p(rh, _dp), outer, _p(pr, _dp), _p(h0, _dp), time, n,
+                                   _p(r, _dp), _p(a, _dp), _p(flags, _u16p), _p(ids, _u32p), _p(deathtime, _dp))
+        return a, flags
+
+    def particle_block(self, m, rh, outer, r_log, h0_log, t, dt, r, v, a, flags, ids, dti=None, deathtime=None):
+        """One CPU time-block; returns dict of updated arrays (inputs are
+        copied).  dti / deathtime are indexed by particle id."""
+        m = np.ascontiguousarray(m, np.float64)
+        rh = np.ascontiguousarray(rh, np.float64)
+        rl = np.ascontiguousarray(r_log, np.float64)
+        h0 = np.ascontiguousarray(h0_log, np.float64).reshape(-1, 3)
+        steps = h0.shape[0]
+        r = np.array(r, np.float64, copy=True)
+        v = np.array(v, np.float64, copy=True)
+        a = np.array(a, np.float64, copy=True)
+        flags = np.array(flags, np.uint16, copy=True)
+        ids = np.ascontiguousarray(ids, np.uint32)
+        n = r.shape[0]
+        size = max(n, int(ids.max()) + 1 if n else 0)
+        dti = np.zeros(size, np.uint32) if dti is None else np.array(dti, np.uint32, copy=True)
+        if deathtime is None:
+            deathtime = np.full(size, np.nan)
+        bad = C.c_uint64(0)
+        rc = self._f("particle_block")(len(m), _p(m, _dp), _p(rh, _dp), outer, _p(rl, _dp), _p(h0, _dp), steps,
+                                       t, dt, n, _p(r, _dp), _p(v, _dp), _p(a, _dp), _p(flags, _u16p),
+                                       _p(ids, _u32p), _p(dti, _u32p), _p(deathtime, _dp), C.byref(bad))
+        return {"rc": rc, "r": r, "v": v, "a": a, "flags": flags, "dti": dti, "deathtime": deathtime,
+                "bad": bad.value}
+
+    def drift(self, t, r, v, mu, fixed=True):
+        r = np.array(r, np.float64, copy=True)
+        v = np.array(v, np.float64, copy=True)
+        n = r.shape[0]
+        mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, np.float64), (n,)))
+        bad = C.c_uint64(0)
+        rc = self._f("drift")(t, n, _p(r, _dp), _p(v, _dp), _p(mu, _dp), int(fixed), C.byref(bad))
+        return rc, r, v
+
+    # -- planets ----------------------------------------------------------
+    def planets(self, m, r, v, ids=None, cull_radius=0.5, rh_factor=0.0, outer=300.0):
+        return OraclePlanets(self, m, r, v, ids, cull_radius, rh_factor, outer)
+
+    def load_h0(self, m, r_log):
+        m = np.ascontiguousarray(m, np.float64)
+        rl = np.ascontiguousarray(r_log, np.float64)
+        steps = rl.shape[0]
+        out = np.empty((steps, 3))
+        self._f("load_h0")(len(m), _p(m, _dp), _p(rl, _dp), steps, _p(out, _dp))
+        return out
+
+    def planet_metrics(self, m, r, v):
+        m = np.ascontiguousarray(m, np.float64)
+        r = np.ascontiguousarray(r, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        e = np.empty(1)
+        lv = np.empty(3)
+        self._f("planet_metrics")(len(m), _p(m, _dp), _p(r, _dp), _p(v, _dp), _p(e, _dp), _p(lv, _dp))
+        return float(e[0]), lv
+
+    def stable_partition(self, flags, begin, length, mode):
+        flags = np.ascontiguousarray(flags, np.uint16)
+        perm = np.empty(length, np.uint64)
+        split = self._f("stable_partition")(_p(flags, _u16p), begin, length, mode, _p(perm, _u64p))
+        return int(split), perm
+
+    def to_elements(self, mu, r, v):
+        r = np.ascontiguousarray(r, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.empty(6)
+        self._f("to_elements")(mu, _p(r, _dp), _p(v, _dp), _p(out, _dp))
+        return out
+
+    def from_elements(self, mu, el):
+        el = np.ascontiguousarray(el, np.float64)
+        r, v = np.empty(3), np.empty(3)
+        self._f("from_elements")(mu, _p(el, _dp), _p(r, _dp), _p(v, _dp))
+        return r, v
+
+
+class OraclePlanets:
+    def __init__(self, o: Oracle, m, r, v, ids, cull_radius, rh_factor, outer):
+        self.o = o
+        self.m = np.ascontiguousarray(m, np.float64)
+        self.n = len(self.m)
+        r = np.ascontiguousarray(r, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        self.ids = np.arange(self.n, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, np.uint32)
+        self.h0_0 = np.empty(3)
+        self.h = o._f("planets_new")(self.n, _p(self.m, _dp), _p(r, _dp), _p(v, _dp), _p(self.ids, _u32p),
+                                     cull_radius, rh_factor, outer, _p(self.h0_0, _dp))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.o._f("planets_free")(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def block(self, n_steps, dt):
+        rl = np.empty((n_steps, self.n - 1, 3))
+        vl = np.empty((n_steps, self.n - 1, 3))
+        h0 = np.empty((n_steps, 3))
+        rc = self.o._f("planets_block")(self.h, n_steps, dt, _p(rl, _dp), _p(vl, _dp), _p(h0, _dp))
+        if rc != 0:
+            raise OracleError(rc, self.o.err())
+        return rl, vl, h0
+
+    def state(self):
+        r, v, rh = np.empty((self.n, 3)), np.empty((self.n, 3)), np.empty(self.n)
+        self.o._f("planets_get")(self.h, _p(r, _dp), _p(v, _dp), _p(rh, _dp))
+        return r, v, rh
+
+    def recalculate_rh(self):
+        self.o._f("planets_recalculate_rh")(self.h)

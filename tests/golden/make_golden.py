@@ -1,0 +1,121 @@
+"""Write the golden vectors of the CPU particle path.
+
+Every expected output here comes from the reference's own CPU sources
+(src/wh.cpp, src/convert.cpp, src/data.cpp), compiled by oracle/Makefile
+into oracle/_ref/libglisser_ref.so in the container that holds the
+reference tree.  Inputs are seeded synthetic systems.  Run from the repo
+root:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+The fixtures are small .npz files (inputs and expected outputs only).
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from glisser_amd import systems  # noqa: E402
+from oracle.pyoracle import Oracle  # noqa: E402
+from tests.helpers import oracle_run, planet_blocks  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+DT = 180.0
+OUTER = 300.0
+
+
+def planets(ref):
+    s = systems.outer_solar_system(1, seed=1)
+    pl = ref.planets(s.m, s.r, s.v)
+    blocks = [pl.block(256, DT) for _ in range(2)]
+    r_end, v_end, rh = pl.state()
+    e, lv = ref.planet_metrics(s.m, r_end, v_end)
+    np.savez_compressed(OUT / "planets.npz", m=s.m, r=s.r, v=s.v, h0_0=pl.h0_0,
+                        r_log=np.stack([b[0] for b in blocks]), v_log=np.stack([b[1] for b in blocks]),
+                        h0_log=np.stack([b[2] for b in blocks]), r_end=r_end, v_end=v_end, rh=rh,
+                        energy_end=e, l_end=lv, dt=DT)
+
+
+def particle_block(ref):
+    s = systems.outer_solar_system(64, seed=2)
+    pl = ref.planets(s.m, s.r, s.v)
+    rh = pl.state()[2]
+    a0, f0 = ref.particle_accels(s.m, rh, OUTER, s.r[1:], pl.h0_0, s.pr, np.zeros(s.n), s.ids)
+    rl, _vl, h0 = pl.block(512, DT)
+    out = ref.particle_block(s.m, rh, OUTER, rl, h0, 0.0, DT, s.pr, s.pv, a0, f0, s.ids)
+    assert out["rc"] == 0
+    np.savez_compressed(OUT / "particle_block.npz", m=s.m, rh=rh, outer=OUTER, r_log=rl, h0_log=h0, dt=DT,
+                        r=s.pr, v=s.pv, a=a0, flags=f0, ids=s.ids, h0_0=pl.h0_0, planet_r0=s.r[1:],
+                        r_out=out["r"], v_out=out["v"], a_out=out["a"], flags_out=out["flags"],
+                        dti_out=out["dti"])
+
+
+def long_run(ref):
+    """64 particles x 49 blocks x 2048 steps with the executor's partition
+    between blocks; final state only (planet logs are regenerated by the
+    bit-exact planet producer).  The regular-particle mask compares against
+    a run from initial positions nudged by one ulp."""
+    import copy
+    s = systems.outer_solar_system(64, seed=7)
+    h0_0, rh0, blocks = planet_blocks(ref, s, 49, 2048, DT)
+    out = oracle_run(ref, s, h0_0, rh0, blocks, DT, OUTER)
+    s2 = copy.deepcopy(s)
+    s2.pr = np.nextafter(s.pr, np.inf)
+    out2 = oracle_run(ref, s2, h0_0, rh0, blocks, DT, OUTER)
+    o1, o2 = np.argsort(out["id"]), np.argsort(out2["id"])
+    div = np.linalg.norm(out2["r"][o2] - out["r"][o1], axis=1) / np.linalg.norm(out["r"][o1], axis=1)
+    np.savez_compressed(OUT / "long_run.npz", seed=7, n=64, n_blocks=49, tbsize=2048, dt=DT, outer=OUTER,
+                        r0=s.pr, v0=s.pv, m=s.m, pl_r=s.r, pl_v=s.v,
+                        r=out["r"], v=out["v"], a=out["a"], flags=out["flags"], ids=out["id"],
+                        dti_by_id=out["dti_by_id"], n_alive=out["n_alive"], divergence=div)
+
+
+def encounters(ref):
+    s = systems.three_body(2000, seed=5, a_range=(20.0, 40.0), e_hi=0.6, i_hi_deg=5.0)
+    h0_0, rh0, blocks = planet_blocks(ref, s, 3, 512, 30.0, cull_radius=3.0, rh_factor=3.5)
+    out = oracle_run(ref, s, h0_0, rh0, blocks, 30.0, OUTER)
+    np.savez_compressed(OUT / "encounters.npz", seed=5, n=2000, m=s.m, pl_r=s.r, pl_v=s.v, r0=s.pr, v0=s.pv,
+                        flags=out["flags"], ids=out["id"], dti_by_id=out["dti_by_id"], n_alive=out["n_alive"],
+                        r=out["r"], v=out["v"])
+
+
+def partition(ref):
+    rng = np.random.default_rng(5)
+    n = 3000
+    flags = rng.choice(np.array([0, 0, 0, 0x0201, 0x0080, 0x0002, 0x0001, 0x0004, 0x0100], np.uint16), n)
+    s0, p0 = ref.stable_partition(flags, 0, n, 0)
+    s1, p1 = ref.stable_partition(flags, 100, n - 100, 1)
+    np.savez_compressed(OUT / "partition.npz", flags=flags, split_unflagged=s0, perm_unflagged=p0,
+                        split_alive_from100=s1, perm_alive_from100=p1)
+
+
+def elements(ref):
+    rng = np.random.default_rng(6)
+    mu = 2.959122e-4
+    el = np.column_stack([rng.uniform(1, 80, 200), rng.uniform(0, 0.9, 200), rng.uniform(0, np.pi, 200),
+                          rng.uniform(0, 2 * np.pi, 200), rng.uniform(0, 2 * np.pi, 200),
+                          rng.uniform(-np.pi, np.pi, 200)])
+    el[:5, 1] = 0.0
+    el[5:10, 2] = 0.0
+    rv = [ref.from_elements(mu, e) for e in el]
+    r = np.array([x[0] for x in rv])
+    v = np.array([x[1] for x in rv])
+    back = np.array([ref.to_elements(mu, r[k], v[k]) for k in range(len(el))])
+    np.savez_compressed(OUT / "elements.npz", mu=mu, el=el, r=r, v=v, el_back=back)
+
+
+if __name__ == "__main__":
+    ref = Oracle(reference=True)
+    planets(ref)
+    particle_block(ref)
+    long_run(ref)
+    encounters(ref)
+    partition(ref)
+    elements(ref)
+    for f in sorted(OUT.glob("*.npz")):
+        print(f.name, f.stat().st_size)
