@@ -10,6 +10,7 @@
 // main stream uses events instead of host synchronisation.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -181,6 +182,7 @@ int fill_consts(uint32_t n_bodies, const double* m, const uint32_t* id, const do
         pc->m[j] = m[j];
         pc->rh2[j] = rh[j] * rh[j];
         pc->id[j] = id ? id[j] : j;
+        pc->mf[j] = (float)m[j];
     }
     return GLX_OK;
 }
@@ -357,7 +359,8 @@ int glx_upload_planet_block(glx_ctx* c, int buf, uint32_t n_bodies, const double
 int glx_run_block(glx_ctx* c, int buf, double dt, double outer_radius, int form) {
     if (!c) return fail(GLX_EINVAL, "null context");
     if (buf != 0 && buf != 1) return fail(GLX_EINVAL, "planet buffer %d is not 0 or 1", buf);
-    if (form != GLX_FORM_FP64) return fail(GLX_EINVAL, "kernel form %d is not available", form);
+    if (form != GLX_FORM_FP64 && form != GLX_FORM_FAST && form != GLX_FORM_MIXED)
+        return fail(GLX_EINVAL, "kernel form %d is not available", form);
     PlanetBuf& b = c->pb[buf];
     if (!b.valid) return fail(GLX_ESTATE, "planet buffer %d holds no block", buf);
     if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
@@ -375,6 +378,9 @@ int glx_run_block(glx_ctx* c, int buf, double dt, double outer_radius, int form)
     A.dt = dt;
     A.half_dt = dt / 2;
     A.mu = b.mu;
+    A.inv_mu = 1.0 / b.mu;
+    A.sqrt_mu = std::sqrt(b.mu);
+    A.inv_sqrt_mu = 1.0 / A.sqrt_mu;
     A.outer2 = outer_radius * outer_radius;
     A.pc = b.pc;
     GLX_HIP(hipStreamWaitEvent(c->main_s, b.uploaded, 0));

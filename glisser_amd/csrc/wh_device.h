@@ -34,6 +34,7 @@ struct PlanetConsts {
     double m[GLX_MAX_BODIES];    // GM of each body, m[0] = Sun
     double rh2[GLX_MAX_BODIES];  // squared encounter radius (rh[0]: Sun cull radius)
     uint32_t id[GLX_MAX_BODIES]; // planet id written into the flag high byte
+    float mf[GLX_MAX_BODIES];    // GM in FP32 (dominant-term selection only)
 };
 
 // Arguments of one time-block launch.  Particle arrays are SoA; `log` is
@@ -60,6 +61,9 @@ struct BlockArgs {
     double half_dt;
     double mu;
     double outer2;
+    double inv_mu;       // 1/mu, sqrt(mu), 1/sqrt(mu): FAST / MIXED drift
+    double sqrt_mu;
+    double inv_sqrt_mu;
     PlanetConsts pc;
 };
 

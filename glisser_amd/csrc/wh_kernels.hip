@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "wh_device.h"
+#include "wh_fast.h"
 #include "launch.h"
 
 namespace glx {
@@ -48,10 +49,20 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         for (uint32_t s = 0; s < cn; s++) {
             if (p.flags == 0) {
                 const double* rowp = lds + s * row;
-                kick(p, A.half_dt);
-                drift_fp64(p, A.dt, A.mu);
-                accel_fp64<NB>(p, rowp, A, nb);
-                kick(p, A.half_dt);
+                if constexpr (FORM == GLX_FORM_FP64) {
+                    kick(p, A.half_dt);
+                    drift_fp64(p, A.dt, A.mu);
+                    accel_fp64<NB>(p, rowp, A, nb);
+                    kick(p, A.half_dt);
+                } else {
+                    kick_fma(p, A.half_dt);
+                    drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+                    if constexpr (FORM == GLX_FORM_MIXED)
+                        accel_mixed<NB>(p, rowp, A, nb);
+                    else
+                        accel_fast<NB>(p, rowp, A, nb);
+                    kick_fma(p, A.half_dt);
+                }
                 dti = cs + s + 1;
             }
         }
@@ -124,6 +135,10 @@ hipError_t launch_block(const BlockArgs& A, int form, hipStream_t stream) {
     switch (form) {
         case GLX_FORM_FP64:
             return launch_block_form<GLX_FORM_FP64>(A, stream);
+        case GLX_FORM_FAST:
+            return launch_block_form<GLX_FORM_FAST>(A, stream);
+        case GLX_FORM_MIXED:
+            return launch_block_form<GLX_FORM_MIXED>(A, stream);
         default:
             return hipErrorInvalidValue;
     }
