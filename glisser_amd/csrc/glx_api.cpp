@@ -383,6 +383,9 @@ int glx_run_block(glx_ctx* c, int buf, double dt, double outer_radius, int form)
     A.inv_sqrt_mu = 1.0 / A.sqrt_mu;
     A.outer2 = outer_radius * outer_radius;
     A.pc = b.pc;
+    A.dominant = 1;
+    for (uint32_t j = 2; j < b.n_bodies; j++)
+        if (b.pc.m[j] > b.pc.m[A.dominant]) A.dominant = j;
     GLX_HIP(hipStreamWaitEvent(c->main_s, b.uploaded, 0));
     GLX_HIP(hipEventRecord(c->k0, c->main_s));
     GLX_HIP(glx::launch_block(A, form, c->main_s));

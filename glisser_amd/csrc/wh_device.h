@@ -57,6 +57,7 @@ struct BlockArgs {
     uint32_t n_steps;
     uint32_t chunk;     // steps staged in LDS at a time
     uint32_t n_bodies;  // used by the generic (NB == 0) instantiation
+    uint32_t dominant;  // MIXED form: planet whose term stays FP64 (heaviest)
     double dt;
     double half_dt;
     double mu;

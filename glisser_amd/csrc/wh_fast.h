@@ -177,16 +177,15 @@ __device__ __forceinline__ void accel_fast(Particle& p, const double* __restrict
     if (rad > A.outer2) p.flags = kFlagOut;
 }
 
-// Planet sum, MIXED form: FP32 r^-3 per planet, FP64 accumulation, and one
-// FP64 Newton refinement of the largest |term| (the lane's dominant planet).
+// Planet sum, MIXED form: FP32 r^-3 for every planet except the dominant
+// one (the heaviest, chosen on the host), whose term is evaluated with the
+// refined FP64 reciprocal square root; all terms accumulate in FP64 and
+// |dr|^2 and the encounter test stay FP64.
 template <int NB>
 __device__ __forceinline__ void accel_mixed(Particle& p, const double* __restrict__ row, const BlockArgs& A,
                                             uint32_t nb) {
     double ax = row[0], ay = row[1], az = row[2];
     const uint32_t nbodies = NB > 0 ? (uint32_t)NB : nb;
-    float best = -1.0f;
-    float best_y = 0.0f;
-    double best_fac = 0.0, best_rad = 1.0, bdx = 0.0, bdy = 0.0, bdz = 0.0, best_m = 0.0;
 #pragma unroll
     for (uint32_t j = 1; j < (NB > 0 ? (uint32_t)NB : (uint32_t)GLX_MAX_BODIES); j++) {
         if (NB == 0 && j >= nbodies) break;
@@ -195,34 +194,18 @@ __device__ __forceinline__ void accel_mixed(Particle& p, const double* __restric
         const double dz = p.rz - row[3 * j + 2];
         const double rad = fma(dz, dz, fma(dy, dy, dx * dx));
         if (rad < A.pc.rh2[j] && p.flags == 0) p.flags = ((A.pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
-        const float yf = __builtin_amdgcn_rsqf((float)rad);
-        const float y2f = yf * yf;
-        const double facd = A.pc.m[j] * (double)(y2f * yf);  // m_j |dr|^-3, FP32 r^-3
-        const float mag = A.pc.mf[j] * y2f;                   // |term| = m_j / |dr|^2
-        ax = fma(-dx, facd, ax);
-        ay = fma(-dy, facd, ay);
-        az = fma(-dz, facd, az);
-        if (mag > best) {
-            best = mag;
-            best_fac = facd;
-            best_y = yf;
-            best_rad = rad;
-            bdx = dx;
-            bdy = dy;
-            bdz = dz;
-            best_m = A.pc.m[j];
+        double y3;
+        if (j == A.dominant) {  // wave-uniform branch
+            const double y = rsqrt_nr(rad);
+            y3 = y * y * y;
+        } else {
+            const float yf = __builtin_amdgcn_rsqf((float)rad);
+            y3 = (double)(yf * yf * yf);
         }
-    }
-    if (best >= 0.0f) {
-        // FP64 Newton step on the dominant planet's 1/sqrt(rad), then swap
-        // its FP32 term for the refined one.
-        double y = (double)best_y;
-        const double e = fma(-0.5 * best_rad, y * y, 0.5);
-        y = fma(y, e, y);
-        const double corr = best_m * (y * y * y) - best_fac;
-        ax = fma(-bdx, corr, ax);
-        ay = fma(-bdy, corr, ay);
-        az = fma(-bdz, corr, az);
+        const double fac = A.pc.m[j] * y3;
+        ax = fma(-dx, fac, ax);
+        ay = fma(-dy, fac, ay);
+        az = fma(-dz, fac, az);
     }
     p.ax = ax;
     p.ay = ay;
