@@ -7,13 +7,16 @@
 //     divisions and square roots; 1/a = -2E/mu needs no division at all;
 //   * reciprocals / reciprocal square roots start from v_rcp_f64 /
 //     v_rsq_f64 and take two FMA Newton steps;
-//   * sin/cos of the mean-anomaly step and of the first guess use a Taylor
-//     polynomial (|x| <= 0.8; the device library beyond that);
-//   * Newton corrections rotate (sin, cos) by the small correction with a
-//     short series instead of re-evaluating sin/cos, and the iteration stops
-//     once every lane of the wave has |delta| < 1e-14 (the early-exit mode
-//     the reference offers under CUDA_KEPEQ_CHECK_CONVERGENCE,
-//     src/wh.cu:60-71), at most five iterations;
+//   * sin/cos of the mean-anomaly step use a five-term Taylor polynomial
+//     when every lane of the wave has |x| <= 0.1 (the device library
+//     otherwise); the first guess and every Newton correction rotate
+//     (sin E, cos E - 1) by the small angle with a short series instead of
+//     re-evaluating sin/cos; cos E - 1 and E - dM are carried explicitly, so
+//     f, g and the Kepler residual avoid cancellation;
+//   * the Newton iteration stops once every lane of the wave has
+//     |delta| < 1e-14 after applying it (the early-exit mode the reference
+//     offers under CUDA_KEPEQ_CHECK_CONVERGENCE, src/wh.cu:60-71), at most
+//     five iterations;
 //   * FMA contraction is allowed.
 // MIXED additionally evaluates each planet's r^-3 with the FP32 reciprocal
 // square root (|dr|^2 and the encounter test stay FP64), accumulates the
@@ -45,53 +48,61 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
     return fma(y, e, y);
 }
 
-// sin and cos by Taylor series in x^2 (terms through x^19 / x^18), accurate
-// to below 1e-17 absolute for |x| <= 0.8; larger arguments (any lane) take
-// the device library.
-__device__ __forceinline__ void sincos_small(double x, double* s, double* c) {
-    if (fabs(x) > 0.8) {
-        sincos(x, s, c);
-        return;
-    }
+// Taylor coefficients of sin (x^3..x^11) and of cos - 1 (x^2..x^10):
+// (-1)^k / n!.  Kept in constant memory so they reach the FMAs as scalar
+// operands (literal FP64 constants would be copied into VGPRs first).
+__constant__ double kSinTaylor[5] = {-1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0 / 362880.0, -1.0 / 39916800.0};
+__constant__ double kCosTaylor[5] = {-1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0, -1.0 / 3628800.0};
+
+// sin x and cos x - 1 from the first N Taylor terms (Horner in x^2).
+template <int N>
+__device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) {
     const double x2 = x * x;
-    // sin: x * (1 - x2/3! + x2^2/5! - ... + x2^9/19!)
-    double ps = -1.0 / 121645100408832000.0;  // -1/19!
-    ps = fma(ps, x2, 1.0 / 355687428096000.0);  // 1/17!
-    ps = fma(ps, x2, -1.0 / 1307674368000.0);   // -1/15!
-    ps = fma(ps, x2, 1.0 / 6227020800.0);       // 1/13!
-    ps = fma(ps, x2, -1.0 / 39916800.0);        // -1/11!
-    ps = fma(ps, x2, 1.0 / 362880.0);           // 1/9!
-    ps = fma(ps, x2, -1.0 / 5040.0);            // -1/7!
-    ps = fma(ps, x2, 1.0 / 120.0);              // 1/5!
-    ps = fma(ps, x2, -1.0 / 6.0);               // -1/3!
-    *s = fma(ps * x2, x, x);
-    // cos: 1 - x2/2! + x2^2/4! - ... + x2^9/18!
-    double pc = 1.0 / 6402373705728000.0;       // 1/18!
-    pc = fma(pc, x2, -1.0 / 20922789888000.0);  // -1/16!
-    pc = fma(pc, x2, 1.0 / 87178291200.0);      // 1/14!
-    pc = fma(pc, x2, -1.0 / 479001600.0);       // -1/12!
-    pc = fma(pc, x2, 1.0 / 3628800.0);          // 1/10!
-    pc = fma(pc, x2, -1.0 / 40320.0);           // -1/8!
-    pc = fma(pc, x2, 1.0 / 720.0);              // 1/6!
-    pc = fma(pc, x2, -1.0 / 24.0);              // -1/4!
-    pc = fma(pc, x2, 0.5);                      // 1/2!  (applied with a minus)
-    *c = fma(-pc, x2, 1.0);
+    double ps = kSinTaylor[N - 1];
+    double pc = kCosTaylor[N - 1];
+#pragma unroll
+    for (int k = N - 2; k >= 0; k--) {
+        ps = fma(ps, x2, kSinTaylor[k]);
+        pc = fma(pc, x2, kCosTaylor[k]);
+    }
+    s = fma(ps * x2, x, x);
+    cm1 = pc * x2;
 }
 
-// Rotate (s, c) = (sin E, cos E) to E + d for |d| < 1e-4:
-// sin d = d - d^3/6 + d^5/120, cos d = 1 - d^2/2 + d^4/24 (next terms
-// below 1e-26).  Larger corrections re-evaluate sin/cos of the new angle.
-__device__ __forceinline__ void rotate_small(double e_new, double d, double& s, double& c) {
-    if (fabs(d) >= 1e-4) {
-        sincos_small(e_new, &s, &c);
+// Device-library sin/cos for large arguments, kept out of line so its
+// range-reduction constants do not occupy registers in the hot loop.
+__device__ __noinline__ void sincos_library(double x, double* s, double* cm1) {
+    double c;
+    sincos(x, s, &c);
+    *cm1 = c - 1.0;
+}
+
+// sin x and cos x - 1: five Taylor terms when every lane of the wave has
+// |x| <= 0.1 (truncation below 1e-20 absolute), the device library
+// otherwise.
+__device__ __forceinline__ void sincos_m1(double x, double& s, double& cm1) {
+    if (__all(fabs(x) <= 0.1)) {
+        sincos_taylor<5>(x, s, cm1);
+    } else {
+        sincos_library(x, &s, &cm1);
+    }
+}
+
+// Advance (s, cm1) = (sin E, cos E - 1) to E + d.  sin d and cos d - 1 come
+// from four Taylor terms each, exact to rounding for |d| <= 0.02; when a
+// lane's correction is larger the wave re-evaluates sin/cos of e_new.
+__device__ __forceinline__ void rotate_m1(double e_new, double d, double& s, double& cm1) {
+    if (!__all(fabs(d) <= 0.02)) {
+        sincos_m1(e_new, s, cm1);
         return;
     }
-    const double d2 = d * d;
-    const double sd = d * fma(d2, fma(d2, 1.0 / 120.0, -1.0 / 6.0), 1.0);
-    const double cm1 = d2 * fma(d2, 1.0 / 24.0, -0.5);  // cos d - 1
-    const double s0 = s, c0 = c;
-    s = fma(s0, cm1, fma(c0, sd, s0));
-    c = fma(c0, cm1, fma(-s0, sd, c0));
+    double sd, cd;
+    sincos_taylor<4>(d, sd, cd);
+    // sin(E+d) = s (1 + cd) + (1 + cm1) sd ; cos(E+d) - 1 = cm1 + cd + cm1 cd - s sd
+    const double s_new = s + sd + fma(s, cd, cm1 * sd);
+    const double c_new = cm1 + cd + fma(cm1, cd, -s * sd);
+    s = s_new;
+    cm1 = c_new;
 }
 
 // Kepler drift, FAST form.  inv_mu, sqrt_mu, inv_sqrt_mu: host constants.
@@ -107,35 +118,42 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
         p.flags = kFlagUnbound;
         return;
     }
-    const double ia = -2.0 * energy * inv_mu;    // 1/a
+    const double ia = -2.0 * energy * inv_mu;       // 1/a
     const double a = rcp_nr(ia);
-    const double rs_ia = rsqrt_nr(ia);            // sqrt(a)
-    const double n_ = sqrt_mu * ia * (ia * rs_ia);  // sqrt(mu / a^3)
-    const double in_ = inv_sqrt_mu * a * rs_ia;     // 1/n
+    const double rs_ia = rsqrt_nr(ia);               // sqrt(a)
+    const double n_ = sqrt_mu * ia * (ia * rs_ia);   // sqrt(mu / a^3)
+    const double in_ = inv_sqrt_mu * a * rs_ia;      // 1/n
     const double ecosEo = fma(-dist, ia, 1.0);
     const double esinEo = vdotr * in_ * ia * ia;
     const double dtn = dt * n_;
     // the reference's (int)(dt n / (pi/2)) full-turn subtraction; 0 for dt n < pi/2
     const double dM = fma(-(kPi * 2), (double)(int)(dtn * (2.0 / kPi)), dtn);
     const double adv_dt = dM * in_;
-    double sM, cM;
-    sincos_small(dM, &sM, &cM);
-    double dE = dM - esinEo + esinEo * cM + ecosEo * sM;
-    double s, c;
-    sincos_small(dE, &s, &c);
+    double s, cm1;
+    sincos_m1(dM, s, cm1);
+    // first guess dE = dM + esinEo (cos dM - 1) + ecosEo sin dM; dd = dE - dM
+    double dd = fma(esinEo, cm1, ecosEo * s);
+    double dE = dM + dd;
+    rotate_m1(dE, dd, s, cm1);
+    const double omec = 1.0 - ecosEo;
     double delta = 0.0;
     for (int k = 0; k < kKepIts; k++) {
-        const double f = fma(esinEo, 1.0 - c, fma(-ecosEo, s, dE)) - dM;
-        const double fp = fma(esinEo, s, fma(-ecosEo, c, 1.0));
-        delta = -f * rcp_nr(fp);
-        dE += delta;
-        rotate_small(dE, delta, s, c);
-        if (!__any(fabs(delta) >= kTolKep)) break;
+        // F(E) = E - ecosEo sin E + esinEo (1 - cos E) - dM,  F' = 1 - ecosEo cos E + esinEo sin E
+        const double f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
+        const double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
+        // after the first correction the step is tiny; the raw reciprocal
+        // estimate is then accurate enough for it
+        const double ifp = k == 0 ? rcp_nr(fp) : __builtin_amdgcn_rcp(fp);
+        delta = -f * ifp;
+        dd += delta;
+        dE = dM + dd;
+        rotate_m1(dE, delta, s, cm1);
+        // written so that a NaN correction counts as not converged
+        if (k > 0 && !__any(!(fabs(delta) < kTolKep))) break;
     }
-    p.flags = fabs(delta) > kTolKep ? kFlagKepler : 0u;
-    const double fp = fma(esinEo, s, fma(-ecosEo, c, 1.0));
+    p.flags = fabs(delta) <= kTolKep ? 0u : kFlagKepler;
+    const double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     const double ifp = rcp_nr(fp);
-    const double cm1 = c - 1.0;
     const double f = fma(a * cm1, idist, 1.0);
     const double g = fma(s - dE, in_, adv_dt);
     const double fdot = -n_ * s * a * idist * ifp;
