@@ -52,10 +52,10 @@ def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int, n_workgrou
     return state + planets
 
 
-def run_gpu(form: str, steps: int, warmup: int, seed: int):
+def run_gpu(form: str, steps: int, warmup: int, seed: int, n_particles: int = N_PARTICLES):
     from glisser_amd import native, systems
 
-    s = systems.outer_solar_system(N_PARTICLES, seed=seed)
+    s = systems.outer_solar_system(n_particles, seed=seed)
     nb = s.n_bodies
     planets = native.Planets(s.m, s.r, s.v, cull_radius=CULL, rh_factor=0.0, outer_radius=OUTER)
     h0_0 = planets.h0_initial()
@@ -67,7 +67,7 @@ def run_gpu(form: str, steps: int, warmup: int, seed: int):
         rl, _vl, h0 = planets.block(TBSIZE, DT)
         blocks.append((rl, h0, rh))
 
-    ctx = native.Context(N_PARTICLES, nb, TBSIZE)
+    ctx = native.Context(n_particles, nb, TBSIZE)
     ctx.upload(s.pr, s.pv, np.zeros(s.n, np.uint16), s.ids, a=np.zeros((s.n, 3)), dti=np.zeros(s.n, np.uint32))
     ctx.set_counts(s.n, s.n)
     ctx.init_accel(s.m, rh0, s.r[1:], h0_0, OUTER)
@@ -143,6 +143,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=1, help="untimed time-blocks")
     ap.add_argument("--form", default="fp64", choices=["fp64", "fast", "mixed"])
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=N_PARTICLES,
+                    help="swarm size (the benchmark workload is 200,000)")
     ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args(argv)
@@ -153,14 +155,14 @@ def main(argv=None):
         return 2
     rank = int(os.environ.get("RANK", "0"))
 
-    res = run_gpu(args.form, args.steps, args.warmup, args.seed)
+    res = run_gpu(args.form, args.steps, args.warmup, args.seed, args.particles)
     value = res["particle_steps"] / res["elapsed_s"]
     n_planets = res["n_bodies"] - 1
     f_alg = flops_per_particle_step(n_planets)
     k_avg_s = float(np.mean(res["kernel_ms"])) / 1e3
     per_launch = res["particle_steps"] / max(len(res["kernel_ms"]), 1)
     achieved_tflops = per_launch * f_alg / k_avg_s / 1e12
-    workgroups = (N_PARTICLES + 255) // 256
+    workgroups = (args.particles + 255) // 256
     line = {
         "metric": "particle-steps/s at 200k particles: FP64 vs mixed; % FP64 vector peak",
         "value": value,
@@ -176,7 +178,7 @@ def main(argv=None):
         "data": "synthetic (seeded Sun + 4 giants, 200k-particle swarm at 60-65 au)",
         "config": {
             "workload": "wh-timeblock-200k",
-            "n_particles": N_PARTICLES,
+            "n_particles": args.particles,
             "n_bodies": res["n_bodies"],
             "tbsize": TBSIZE,
             "dt_days": DT,
@@ -196,7 +198,7 @@ def main(argv=None):
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
             "traffic": None,
             "flops_per_particle_step": f_alg,
-            "hbm_gbs_algorithmic": hbm_bytes_per_block(N_PARTICLES, res["n_bodies"], TBSIZE, workgroups)
+            "hbm_gbs_algorithmic": hbm_bytes_per_block(args.particles, res["n_bodies"], TBSIZE, workgroups)
             / k_avg_s / 1e9,
         },
     }

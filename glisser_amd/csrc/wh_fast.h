@@ -6,22 +6,25 @@
 //   * one reciprocal square root of |r|^2 and of 1/a replace the drift's
 //     divisions and square roots; 1/a = -2E/mu needs no division at all;
 //   * reciprocals / reciprocal square roots start from v_rcp_f64 /
-//     v_rsq_f64 and take two FMA Newton steps;
-//   * sin/cos of the mean-anomaly step use a five-term Taylor polynomial
-//     when every lane of the wave has |x| <= 0.1 (the device library
-//     otherwise); the first guess and every Newton correction rotate
-//     (sin E, cos E - 1) by the small angle with a short series instead of
-//     re-evaluating sin/cos; cos E - 1 and E - dM are carried explicitly, so
-//     f, g and the Kepler residual avoid cancellation;
-//   * the Newton iteration stops once every lane of the wave has
-//     |delta| < 1e-14 after applying it (the early-exit mode the reference
-//     offers under CUDA_KEPEQ_CHECK_CONVERGENCE, src/wh.cu:60-71), at most
-//     five iterations;
+//     v_rsq_f64 (relative error < 6e-8) and take two FMA Newton steps where
+//     the result feeds the state, none where it only scales a Newton
+//     correction;
+//   * sin/cos of the mean-anomaly step come from a five-term Taylor
+//     polynomial (|dM| <= 0.1); the first guess and each Newton correction
+//     rotate (sin E, cos E - 1) by the small angle with a short series
+//     instead of re-evaluating sin/cos; cos E - 1 and E - dM are carried
+//     explicitly, so f, g and the Kepler residual avoid cancellation;
+//   * exactly two Newton corrections, accepted when the second is below
+//     1e-14 (the convergence test of the reference's early-exit mode,
+//     src/wh.cu:60-71);
+//   * every shortcut is checked per lane; the few lanes outside them
+//     (large steps, slow convergence, unbound orbits) redo the step with the
+//     reference-arithmetic drift, so results and flags stay defined for any
+//     input;
 //   * FMA contraction is allowed.
-// MIXED additionally evaluates each planet's r^-3 with the FP32 reciprocal
-// square root (|dr|^2 and the encounter test stay FP64), accumulates the
-// acceleration in FP64, and refines the lane's dominant planet term with one
-// FP64 Newton step.
+// MIXED additionally evaluates the r^-3 of every planet but the heaviest
+// with the FP32 reciprocal square root (|dr|^2 and the encounter test stay
+// FP64) and accumulates the acceleration in FP64.
 #pragma once
 
 #include "wh_device.h"
@@ -54,7 +57,9 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 __constant__ double kSinTaylor[5] = {-1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0 / 362880.0, -1.0 / 39916800.0};
 __constant__ double kCosTaylor[5] = {-1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0, -1.0 / 3628800.0};
 
-// sin x and cos x - 1 from the first N Taylor terms (Horner in x^2).
+// sin x and cos x - 1 from the first N Taylor terms (Horner in x^2).  With
+// N = 5 the truncation is below 1e-20 for |x| <= 0.1; with N = 4 below
+// 1e-24 for |x| <= 0.02.
 template <int N>
 __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) {
     const double x2 = x * x;
@@ -69,33 +74,9 @@ __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) 
     cm1 = pc * x2;
 }
 
-// Device-library sin/cos for large arguments, kept out of line so its
-// range-reduction constants do not occupy registers in the hot loop.
-__device__ __noinline__ void sincos_library(double x, double* s, double* cm1) {
-    double c;
-    sincos(x, s, &c);
-    *cm1 = c - 1.0;
-}
-
-// sin x and cos x - 1: five Taylor terms when every lane of the wave has
-// |x| <= 0.1 (truncation below 1e-20 absolute), the device library
-// otherwise.
-__device__ __forceinline__ void sincos_m1(double x, double& s, double& cm1) {
-    if (__all(fabs(x) <= 0.1)) {
-        sincos_taylor<5>(x, s, cm1);
-    } else {
-        sincos_library(x, &s, &cm1);
-    }
-}
-
-// Advance (s, cm1) = (sin E, cos E - 1) to E + d.  sin d and cos d - 1 come
-// from four Taylor terms each, exact to rounding for |d| <= 0.02; when a
-// lane's correction is larger the wave re-evaluates sin/cos of e_new.
-__device__ __forceinline__ void rotate_m1(double e_new, double d, double& s, double& cm1) {
-    if (!__all(fabs(d) <= 0.02)) {
-        sincos_m1(e_new, s, cm1);
-        return;
-    }
+// Advance (s, cm1) = (sin E, cos E - 1) to E + d (|d| <= 0.02; larger
+// corrections are routed to the fallback by the caller).
+__device__ __forceinline__ void rotate_taylor(double d, double& s, double& cm1) {
     double sd, cd;
     sincos_taylor<4>(d, sd, cd);
     // sin(E+d) = s (1 + cd) + (1 + cm1) sd ; cos(E+d) - 1 = cm1 + cd + cm1 cd - s sd
@@ -106,65 +87,82 @@ __device__ __forceinline__ void rotate_m1(double e_new, double d, double& s, dou
 }
 
 // Kepler drift, FAST form.  inv_mu, sqrt_mu, inv_sqrt_mu: host constants.
+//
+// Straight-line path for the common case, then one wave vote: lanes for
+// which a shortcut does not hold (|dM| > 0.1, a first guess or first
+// correction above 0.02, a second correction not below 1e-14, an unbound
+// orbit) redo the step with the reference-arithmetic drift from their saved
+// state, which also sets their 0x04 / 0x08 flags.
 __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, double inv_mu, double sqrt_mu,
                                            double inv_sqrt_mu) {
-    const double r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
-    const double v2 = fma(p.vz, p.vz, fma(p.vy, p.vy, p.vx * p.vx));
-    const double vdotr = fma(p.vz, p.rz, fma(p.vy, p.ry, p.vx * p.rx));
+    const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
+    const double vx0 = p.vx, vy0 = p.vy, vz0 = p.vz;
+    const double r2 = fma(rz0, rz0, fma(ry0, ry0, rx0 * rx0));
+    const double v2 = fma(vz0, vz0, fma(vy0, vy0, vx0 * vx0));
+    const double vdotr = fma(vz0, rz0, fma(vy0, ry0, vx0 * rx0));
     const double idist = rsqrt_nr(r2);
     const double dist = r2 * idist;
     const double energy = fma(v2, 0.5, -mu * idist);
-    if (energy >= 0.0) {
-        p.flags = kFlagUnbound;
-        return;
-    }
-    const double ia = -2.0 * energy * inv_mu;       // 1/a
+    const double ia = -2.0 * energy * inv_mu;      // 1/a
     const double a = rcp_nr(ia);
-    const double rs_ia = rsqrt_nr(ia);               // sqrt(a)
-    const double n_ = sqrt_mu * ia * (ia * rs_ia);   // sqrt(mu / a^3)
-    const double in_ = inv_sqrt_mu * a * rs_ia;      // 1/n
+    const double rs_ia = rsqrt_nr(ia);              // sqrt(a)
+    const double n_ = sqrt_mu * ia * (ia * rs_ia);  // sqrt(mu / a^3)
+    const double in_ = inv_sqrt_mu * a * rs_ia;     // 1/n
     const double ecosEo = fma(-dist, ia, 1.0);
     const double esinEo = vdotr * in_ * ia * ia;
-    const double dtn = dt * n_;
-    // the reference's (int)(dt n / (pi/2)) full-turn subtraction; 0 for dt n < pi/2
-    const double dM = fma(-(kPi * 2), (double)(int)(dtn * (2.0 / kPi)), dtn);
-    const double adv_dt = dM * in_;
+    // |dM| <= 0.1 on this path, so no full turns are removed and the time
+    // left to advance, dM / n, is dt itself
+    const double dM = dt * n_;
     double s, cm1;
-    sincos_m1(dM, s, cm1);
-    // first guess dE = dM + esinEo (cos dM - 1) + ecosEo sin dM; dd = dE - dM
+    sincos_taylor<5>(dM, s, cm1);
+    // first guess dE = dM + esinEo (cos dM - 1) + ecosEo sin dM;  dd = dE - dM
     double dd = fma(esinEo, cm1, ecosEo * s);
-    double dE = dM + dd;
-    rotate_m1(dE, dd, s, cm1);
+    bool bad = !(energy < 0.0) || !(fabs(dM) <= 0.1) || !(fabs(dd) <= 0.02);
+    rotate_taylor(dd, s, cm1);
     const double omec = 1.0 - ecosEo;
-    double delta = 0.0;
-    for (int k = 0; k < kKepIts; k++) {
-        // F(E) = E - ecosEo sin E + esinEo (1 - cos E) - dM,  F' = 1 - ecosEo cos E + esinEo sin E
-        const double f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
-        const double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
-        // after the first correction the step is tiny; the raw reciprocal
-        // estimate is then accurate enough for it
-        const double ifp = k == 0 ? rcp_nr(fp) : __builtin_amdgcn_rcp(fp);
-        delta = -f * ifp;
-        dd += delta;
-        dE = dM + dd;
-        rotate_m1(dE, delta, s, cm1);
-        // written so that a NaN correction counts as not converged
-        if (k > 0 && !__any(!(fabs(delta) < kTolKep))) break;
-    }
-    p.flags = fabs(delta) <= kTolKep ? 0u : kFlagKepler;
-    const double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
+    // Newton on F(E) = E - ecosEo sin E + esinEo (1 - cos E) - dM with
+    // F' = 1 - ecosEo cos E + esinEo sin E.  The corrections are small, so
+    // the raw reciprocal estimate (relative error < 5e-8) is enough for them.
+    double f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
+    double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
+    const double d1 = -f * __builtin_amdgcn_rcp(fp);
+    dd += d1;
+    bad |= !(fabs(d1) <= 0.02);
+    rotate_taylor(d1, s, cm1);
+    f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
+    fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
+    const double d2 = -f * __builtin_amdgcn_rcp(fp);
+    dd += d2;
+    bad |= !(fabs(d2) < kTolKep);
+    // |d2| < 1e-14: the first-order rotation is exact to rounding
+    const double s2 = fma(cm1, d2, s + d2);
+    cm1 = fma(-s, d2, cm1);
+    s = s2;
+    const double dE = dM + dd;
+    fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     const double ifp = rcp_nr(fp);
-    const double f = fma(a * cm1, idist, 1.0);
-    const double g = fma(s - dE, in_, adv_dt);
+    const double fg = fma(a * cm1, idist, 1.0);
+    const double g = fma(s - dE, in_, dt);
     const double fdot = -n_ * s * a * idist * ifp;
     const double gdot = fma(cm1, ifp, 1.0);
-    const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
-    p.rx = fma(rx0, f, p.vx * g);
-    p.ry = fma(ry0, f, p.vy * g);
-    p.rz = fma(rz0, f, p.vz * g);
-    p.vx = fma(rx0, fdot, p.vx * gdot);
-    p.vy = fma(ry0, fdot, p.vy * gdot);
-    p.vz = fma(rz0, fdot, p.vz * gdot);
+    p.rx = fma(rx0, fg, vx0 * g);
+    p.ry = fma(ry0, fg, vy0 * g);
+    p.rz = fma(rz0, fg, vz0 * g);
+    p.vx = fma(rx0, fdot, vx0 * gdot);
+    p.vy = fma(ry0, fdot, vy0 * gdot);
+    p.vz = fma(rz0, fdot, vz0 * gdot);
+    p.flags = 0;
+    if (__any(bad)) {
+        if (bad) {
+            p.rx = rx0;
+            p.ry = ry0;
+            p.rz = rz0;
+            p.vx = vx0;
+            p.vy = vy0;
+            p.vz = vz0;
+            drift_fp64(p, dt, mu);
+        }
+    }
 }
 
 // Planet sum, FAST form: r^-3 from one refined FP64 reciprocal square root.
