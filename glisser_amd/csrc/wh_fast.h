@@ -14,9 +14,10 @@
 //     rotate (sin E, cos E - 1) by the small angle with a short series
 //     instead of re-evaluating sin/cos; cos E - 1 and E - dM are carried
 //     explicitly, so f, g and the Kepler residual avoid cancellation;
-//   * exactly two Newton corrections, accepted when the second is below
-//     1e-14 (the convergence test of the reference's early-exit mode,
-//     src/wh.cu:60-71);
+//   * the Kepler solve starts from one Newton step off E = dM (instead of
+//     the reference's series guess) and takes exactly two more corrections,
+//     accepted when the last is below 1e-14 (the convergence test of the
+//     reference's early-exit mode, src/wh.cu:60-71);
 //   * every shortcut is checked per lane; the few lanes outside them
 //     (large steps, slow convergence, unbound orbits) redo the step with the
 //     reference-arithmetic drift, so results and flags stay defined for any
@@ -39,6 +40,12 @@ __device__ __forceinline__ double rcp_nr(double x) {
     y = fma(y, e, y);
     e = fma(-x, y, 1.0);
     return fma(y, e, y);
+}
+
+// Reciprocal with one Newton step (relative error below 3e-15).
+__device__ __forceinline__ double rcp_nr1(double x) {
+    const double y = __builtin_amdgcn_rcp(x);
+    return fma(y, fma(-x, y, 1.0), y);
 }
 
 // 1/sqrt(x) for x > 0, two Newton steps from the hardware estimate.
@@ -89,8 +96,8 @@ __device__ __forceinline__ void rotate_taylor(double d, double& s, double& cm1) 
 // Kepler drift, FAST form.  inv_mu, sqrt_mu, inv_sqrt_mu: host constants.
 //
 // Straight-line path for the common case, then one wave vote: lanes for
-// which a shortcut does not hold (|dM| > 0.1, a first guess or first
-// correction above 0.02, a second correction not below 1e-14, an unbound
+// which a shortcut does not hold (|dM| > 0.1, a start above 0.02, a first
+// correction above 1e-5, a second correction not below 1e-14, an unbound
 // orbit) redo the step with the reference-arithmetic drift from their saved
 // state, which also sets their 0x04 / 0x08 flags.
 __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, double inv_mu, double sqrt_mu,
@@ -115,20 +122,35 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     const double dM = dt * n_;
     double s, cm1;
     sincos_taylor<5>(dM, s, cm1);
-    // first guess dE = dM + esinEo (cos dM - 1) + ecosEo sin dM;  dd = dE - dM
-    double dd = fma(esinEo, cm1, ecosEo * s);
+    const double omec = 1.0 - ecosEo;
+    // Kepler residual F(E) = E - ecosEo sin E + esinEo (1 - cos E) - dM and
+    // F'(E) = 1 - ecosEo cos E + esinEo sin E, written with dd = E - dM and
+    // (s, cm1) = (sin E, cos E - 1).  The start is one Newton step from
+    // E = dM (error ~ e^3 dM^2 / 2) instead of the reference's series guess
+    // (error ~ e^2 dM), so two more corrections reach the 1e-14 tolerance.
+    double f = fma(-esinEo, cm1, -ecosEo * s);
+    double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
+    double dd = -f * rcp_nr(fp);
     bool bad = !(energy < 0.0) || !(fabs(dM) <= 0.1) || !(fabs(dd) <= 0.02);
     rotate_taylor(dd, s, cm1);
-    const double omec = 1.0 - ecosEo;
-    // Newton on F(E) = E - ecosEo sin E + esinEo (1 - cos E) - dM with
-    // F' = 1 - ecosEo cos E + esinEo sin E.  The corrections are small, so
-    // the raw reciprocal estimate (relative error < 5e-8) is enough for them.
-    double f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
-    double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
-    const double d1 = -f * __builtin_amdgcn_rcp(fp);
+    // first correction: the reciprocal with one Newton step (relative error
+    // ~2e-15) is exact enough for a correction of at most 1e-5
+    f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
+    fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
+    const double d1 = -f * rcp_nr1(fp);
     dd += d1;
-    bad |= !(fabs(d1) <= 0.02);
-    rotate_taylor(d1, s, cm1);
+    bad |= !(fabs(d1) <= 1e-5);
+    {
+        // sin d1 = d1 - d1^3/6, cos d1 - 1 = -d1^2/2 + d1^4/24 (exact to rounding for |d1| <= 1e-5)
+        const double d2_ = d1 * d1;
+        const double sd = fma(d1 * d2_, -1.0 / 6.0, d1);
+        const double cd = d2_ * fma(d2_, 1.0 / 24.0, -0.5);
+        const double s_new = s + sd + fma(s, cd, cm1 * sd);
+        const double c_new = cm1 + cd + fma(cm1, cd, -s * sd);
+        s = s_new;
+        cm1 = c_new;
+    }
+    // second correction: tiny, the raw reciprocal estimate suffices
     f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
     fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     const double d2 = -f * __builtin_amdgcn_rcp(fp);
