@@ -1,13 +1,19 @@
 """Throughput of the time-block particle integrator on one MI355X.
 
-Workload (one "step" = one time-block): 200,000 test particles around the
-Sun + 4 giant planets (masses 9.55e-4, 2.86e-4, 4.37e-5, 5.15e-5 Msun at
-5.20/9.55/19.2/30.1 au; particles a ~ U[60,65] au, e ~ U[0,0.1],
-i ~ U[0,5 deg]), dt = 180 d, Time-Block-Size 2048.  Each step runs the
-time-block kernel over the alive particles, the encounter-kill transform
-and the stable partition; the next block's planet log is uploaded on the
-second stream while the kernel runs.  Planet logs for every block are
-produced on the host before the timed region (inputs resident).
+Workload: 200,000 test particles around the Sun + 4 giant planets (masses
+9.55e-4, 2.86e-4, 4.37e-5, 5.15e-5 Msun at 5.20/9.55/19.2/30.1 au;
+particles a ~ U[60,65] au, e ~ U[0,0.1], i ~ U[0,5 deg]), dt = 180 d,
+Time-Block-Size 2048.  One bench "step" is one time-block.  The timed
+region is one launch of `--steps` consecutive blocks (all planet logs
+resident in HBM, produced beforehand by the host planet integrator), then
+the encounter-kill transform, the step count and the stable partition.
+
+Every arithmetic form is measured on the same inputs: fp64 (the
+reference's arithmetic, bit-for-bit with the CPU path in every basic
+operation), fast (FP64 throughout, restructured) and mixed (fast + FP32
+r^-3 for the lighter planets).  `value` is the fast form's rate (pure FP64,
+the form whose error against the CPU oracle matches the fp64 form's);
+the others are reported beside it.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build uses one GPU.
 """
@@ -25,14 +31,20 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-GM_SUN = 2.959122e-4
 N_PARTICLES = 200_000
 TBSIZE = 2048
 DT = 180.0
 OUTER = 300.0
 CULL = 0.5
 FP64_PEAK_TFLOPS = 78.6
-HBM_PEAK_GBS = 8000.0
+HEADLINE_FORM = "fast"
+
+# FP64 FLOPs per particle-step counted by rocprofv3 on this build
+# (64 x (ADD_F64 + MUL_F64 + TRANS_F64 + 2 FMA_F64) per wave instruction,
+# summed over the block kernel and divided by its particle-steps); see
+# profiles/wh_block_counters_200k.md.  Used with the live rate to report
+# the counter-based share of FP64 vector peak.
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": None, "mixed": None}
 
 
 def flops_per_particle_step(n_planets: int) -> int:
@@ -44,89 +56,78 @@ def flops_per_particle_step(n_planets: int) -> int:
 
 
 def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int, n_workgroups: int) -> int:
-    """Algorithmic HBM bytes of one time-block launch: particle state in and
-    out (r, v, a: 72 B; flags 2 B in and out; deathtime index 4 B out) plus
-    the planet block read once per workgroup (n_bodies x 24 B per step)."""
-    state = n_particles * (72 + 2) * 2 + n_particles * 4
+    """Algorithmic HBM bytes of one time-block: particle state read and
+    written once (r, v, a: 72 B; flags 2 B; deathtime index 4 B) plus the
+    planet block read once per workgroup (n_bodies x 24 B per step)."""
+    state = n_particles * (72 + 2 + 4) * 2
     planets = n_workgroups * tbsize * n_bodies * 24
     return state + planets
 
 
-def run_gpu(form: str, steps: int, warmup: int, seed: int, n_particles: int = N_PARTICLES):
+def make_inputs(n_particles: int, n_blocks: int, seed: int):
     from glisser_amd import native, systems
 
     s = systems.outer_solar_system(n_particles, seed=seed)
-    nb = s.n_bodies
     planets = native.Planets(s.m, s.r, s.v, cull_radius=CULL, rh_factor=0.0, outer_radius=OUTER)
     h0_0 = planets.h0_initial()
     rh0 = planets.state()[2]
     blocks = []
-    for _ in range(warmup + steps):
+    for _ in range(n_blocks):
         planets.recalculate_rh()
         rh = planets.state()[2]
         rl, _vl, h0 = planets.block(TBSIZE, DT)
         blocks.append((rl, h0, rh))
+    return s, h0_0, rh0, blocks
 
-    ctx = native.Context(n_particles, nb, TBSIZE)
-    ctx.upload(s.pr, s.pv, np.zeros(s.n, np.uint16), s.ids, a=np.zeros((s.n, 3)), dti=np.zeros(s.n, np.uint32))
-    ctx.set_counts(s.n, s.n)
-    ctx.init_accel(s.m, rh0, s.r[1:], h0_0, OUTER)
-    ctx.kill_encounters()
-    ctx.partition()
 
-    kernel_ms = []
-    part_ms = []
-    psteps = 0
-    t0 = None
-    ctx.upload_planet_block(0, s.m, blocks[0][2], blocks[0][0], blocks[0][1])
-    for k in range(warmup + steps):
-        if k == warmup:
-            ctx.sync()
-            t0 = time.perf_counter()
-        buf = k % 2
-        n_alive, _ = ctx.counts()
-        ctx.run_block(buf, DT, OUTER, form)
-        if k + 1 < warmup + steps:
-            nxt = blocks[k + 1]
-            ctx.upload_planet_block(1 - buf, s.m, nxt[2], nxt[0], nxt[1])
+def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_simd: int):
+    from glisser_amd import native
+
+    s, h0_0, rh0, blocks = inputs
+    n = s.n
+    ctx = native.Context(n, s.n_bodies, TBSIZE, n_planet_buffers=warmup + steps)
+    try:
+        ctx.upload(s.pr, s.pv, np.zeros(n, np.uint16), s.ids, a=np.zeros((n, 3)), dti=np.zeros(n, np.uint32))
+        ctx.set_counts(n, n)
+        ctx.init_accel(s.m, rh0, s.r[1:], h0_0, OUTER)
         ctx.kill_encounters()
-        done = ctx.sum_steps(n_alive)
         ctx.partition()
-        km, pm = ctx.last_times()
-        if k >= warmup:
-            kernel_ms.append(km)
-            part_ms.append(pm)
-            psteps += done
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    n_alive, _ = ctx.counts()
-    ctx.close()
-    return {
-        "elapsed_s": elapsed,
-        "particle_steps": psteps,
-        "kernel_ms": kernel_ms,
-        "partition_ms": part_ms,
-        "n_alive_end": n_alive,
-        "n_bodies": nb,
-    }
+        for k, (rl, h0, rh) in enumerate(blocks[: warmup + steps]):
+            ctx.upload_planet_block(k, s.m, rh, rl, h0)
+        if warmup:
+            ctx.run_blocks(warmup, 0, DT, OUTER, form, shards, waves_per_simd)
+            ctx.kill_encounters()
+            ctx.partition()
+        ctx.sync()
+        n_alive, _ = ctx.counts()
+        t0 = time.perf_counter()
+        ctx.run_blocks(steps, warmup, DT, OUTER, form, shards, waves_per_simd)
+        ctx.kill_encounters()
+        psteps = ctx.sum_steps(n_alive)
+        ctx.partition()
+        ctx.sync()
+        elapsed = time.perf_counter() - t0
+        kernel_ms, part_ms = ctx.last_times()
+        n_end, _ = ctx.counts()
+    finally:
+        ctx.close()
+    return {"elapsed_s": elapsed, "particle_steps": psteps, "kernel_ms": kernel_ms, "partition_ms": part_ms,
+            "n_alive_start": n_alive, "n_alive_end": n_end}
 
 
-def cpu_baseline(seed: int, sample_particles: int):
-    """Reference CPU path (oracle/_ref, the reference's own src/wh.cpp built
-    here) or, where that library is absent, the oracle restatement: one
-    2048-step block over the first `sample_particles` particles of the same
-    swarm on one core."""
-    from glisser_amd import native, systems
+def cpu_baseline(inputs, sample_particles: int):
+    """Reference CPU path (oracle/_ref: the reference's own src/wh.cpp built
+    from its sources) or, where that library is absent, the oracle
+    restatement: one 2048-step block over the first `sample_particles`
+    particles of the same swarm, single thread."""
     from oracle.pyoracle import Oracle, reference_available
 
+    s, h0_0, rh0, blocks = inputs
     kind = "reference" if reference_available() else "port"
     o = Oracle(reference=(kind == "reference"))
-    s = systems.outer_solar_system(N_PARTICLES, seed=seed)
-    pl = native.Planets(s.m, s.r, s.v, cull_radius=CULL, rh_factor=0.0, outer_radius=OUTER)
-    rh = pl.state()[2]
     n = sample_particles
-    a0, f0 = o.particle_accels(s.m, rh, OUTER, s.r[1:], pl.h0_initial(), s.pr[:n], np.zeros(n), s.ids[:n])
-    rl, _vl, h0 = pl.block(TBSIZE, DT)
+    a0, f0 = o.particle_accels(s.m, rh0, OUTER, s.r[1:], h0_0, s.pr[:n], np.zeros(n), s.ids[:n])
+    rl, h0, rh = blocks[0]
     t = time.perf_counter()
     out = o.particle_block(s.m, rh, OUTER, rl, h0, 0.0, DT, s.pr[:n], s.pv[:n], a0, f0, s.ids[:n])
     el = time.perf_counter() - t
@@ -139,12 +140,14 @@ def cpu_baseline(seed: int, sample_particles: int):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="timed time-blocks")
+    ap.add_argument("--steps", type=int, default=10, help="timed time-blocks (one launch)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed time-blocks")
-    ap.add_argument("--form", default="fp64", choices=["fp64", "fast", "mixed"])
+    ap.add_argument("--form", default="all", choices=["all", "fp64", "fast", "mixed"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--particles", type=int, default=N_PARTICLES,
                     help="swarm size (the benchmark workload is 200,000)")
+    ap.add_argument("--shards", type=int, default=4, help="particle slices, one stream each")
+    ap.add_argument("--waves-per-simd", type=int, default=3, help="resident-wave cap (0: none)")
     ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args(argv)
@@ -155,22 +158,39 @@ def main(argv=None):
         return 2
     rank = int(os.environ.get("RANK", "0"))
 
-    res = run_gpu(args.form, args.steps, args.warmup, args.seed, args.particles)
-    value = res["particle_steps"] / res["elapsed_s"]
-    n_planets = res["n_bodies"] - 1
+    inputs = make_inputs(args.particles, args.warmup + args.steps, args.seed)
+    forms = ["fp64", "fast", "mixed"] if args.form == "all" else [args.form]
+    head = HEADLINE_FORM if HEADLINE_FORM in forms else forms[0]
+    n_planets = inputs[0].n_bodies - 1
     f_alg = flops_per_particle_step(n_planets)
-    k_avg_s = float(np.mean(res["kernel_ms"])) / 1e3
-    per_launch = res["particle_steps"] / max(len(res["kernel_ms"]), 1)
-    achieved_tflops = per_launch * f_alg / k_avg_s / 1e12
     workgroups = (args.particles + 255) // 256
+    per_form = {}
+    for form in forms:
+        r = run_gpu(inputs, form, args.steps, args.warmup, args.shards, args.waves_per_simd)
+        block_s = r["kernel_ms"] / 1e3 / args.steps
+        per_block = r["particle_steps"] / args.steps
+        entry = {
+            "value": r["particle_steps"] / r["elapsed_s"],
+            "kernel_ms_per_block": block_s * 1e3,
+            "particle_steps_per_s_kernel": per_block / block_s,
+            "alg_tflops": per_block * f_alg / block_s / 1e12,
+            "partition_ms": r["partition_ms"],
+            "elapsed_s": r["elapsed_s"],
+            "particle_steps": r["particle_steps"],
+        }
+        cf = COUNTER_FLOPS_PER_PSTEP.get(form)
+        if cf:
+            entry["counter_fp64_frac"] = per_block * cf / block_s / 1e12 / FP64_PEAK_TFLOPS
+        per_form[form] = entry
+    h = per_form[head]
     line = {
         "metric": "particle-steps/s at 200k particles: FP64 vs mixed; % FP64 vector peak",
-        "value": value,
+        "value": h["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": res["elapsed_s"] / args.steps * 1e3,
+        "ms_per_step": h["elapsed_s"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -179,31 +199,36 @@ def main(argv=None):
         "config": {
             "workload": "wh-timeblock-200k",
             "n_particles": args.particles,
-            "n_bodies": res["n_bodies"],
+            "n_bodies": inputs[0].n_bodies,
             "tbsize": TBSIZE,
             "dt_days": DT,
-            "form": args.form,
-            "planet_log": "host Jacobi-WH producer, precomputed",
+            "form": head,
+            "blocks_per_launch": args.steps,
+            "shards": args.shards,
+            "waves_per_simd": args.waves_per_simd,
+            "planet_log": "host Jacobi-WH producer, all blocks resident",
             "seed": args.seed,
             "parallelism": "single-gpu",
         },
-        "kernel_ms_avg": k_avg_s * 1e3,
-        "partition_ms_avg": float(np.mean(res["partition_ms"])),
-        "particle_steps_per_s_kernel": per_launch / k_avg_s,
+        "forms": per_form,
         "roofline": {
             "bound": "fp64-valu",
-            "achieved": achieved_tflops,
+            "achieved": h["alg_tflops"],
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+            "frac": h["alg_tflops"] / FP64_PEAK_TFLOPS,
             "traffic": None,
             "flops_per_particle_step": f_alg,
-            "hbm_gbs_algorithmic": hbm_bytes_per_block(args.particles, res["n_bodies"], TBSIZE, workgroups)
-            / k_avg_s / 1e9,
+            "hbm_gbs_algorithmic": hbm_bytes_per_block(args.particles, inputs[0].n_bodies, TBSIZE, workgroups)
+            / (h["kernel_ms_per_block"] / 1e3) / 1e9,
         },
     }
+    if "fp64" in per_form and "mixed" in per_form:
+        line["mixed_vs_fp64"] = per_form["mixed"]["value"] / per_form["fp64"]["value"]
+    if "fp64" in per_form and "fast" in per_form:
+        line["fast_vs_fp64"] = per_form["fast"]["value"] / per_form["fp64"]["value"]
     if rank == 0 and not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.seed, args.cpu_sample)
+        line["cpu_baseline"] = cpu_baseline(inputs, args.cpu_sample)
     if rank == 0:
         print(json.dumps(line), flush=True)
     return 0

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -47,6 +48,9 @@ int fail(int code, const char* fmt, ...) {
             return fail(GLX_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
                         __LINE__);                                                            \
     } while (0)
+
+constexpr uint32_t kMaxShards = 8;       // streams of a multi-block launch
+constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950 LDS per compute unit
 
 struct PlanetBuf {
     double* dev_log = nullptr;  // [tbsize_max][n_bodies_max][3]
@@ -84,7 +88,9 @@ struct glx_ctx {
     unsigned long long* sum_dev = nullptr;
     double* dev_aos = nullptr;       // device staging, cap x 3 doubles
     uint32_t* pin_small = nullptr;   // pinned: totals (2) + sum (2 words)
-    PlanetBuf pb[2];
+    std::vector<PlanetBuf> pb;
+    hipStream_t shard_s[kMaxShards] = {};
+    hipEvent_t shard_done[kMaxShards] = {};
     hipEvent_t k0 = nullptr, k1 = nullptr, q0 = nullptr, q1 = nullptr;
     bool kernel_timed = false, part_timed = false;
     uint64_t n_alive = 0, n_total = 0;
@@ -124,6 +130,8 @@ void release(glx_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->main_s) (void)hipStreamSynchronize(c->main_s);
     if (c->htd_s) (void)hipStreamSynchronize(c->htd_s);
+    for (uint32_t k = 0; k < kMaxShards; k++)
+        if (c->shard_s[k]) (void)hipStreamSynchronize(c->shard_s[k]);
     for (void* p : c->allocs) (void)hipFree(p);
     for (auto& b : c->pb) {
         if (b.pin_log) (void)hipHostFree(b.pin_log);
@@ -133,6 +141,10 @@ void release(glx_ctx* c) {
     if (c->pin_small) (void)hipHostFree(c->pin_small);
     for (hipEvent_t e : {c->k0, c->k1, c->q0, c->q1})
         if (e) (void)hipEventDestroy(e);
+    for (uint32_t k = 0; k < kMaxShards; k++) {
+        if (c->shard_done[k]) (void)hipEventDestroy(c->shard_done[k]);
+        if (c->shard_s[k]) (void)hipStreamDestroy(c->shard_s[k]);
+    }
     if (c->main_s) (void)hipStreamDestroy(c->main_s);
     if (c->htd_s) (void)hipStreamDestroy(c->htd_s);
     delete c;
@@ -240,6 +252,11 @@ int glx_create(const glx_params* p, glx_ctx** out) {
     if (hipSetDevice(c->device) != hipSuccess) return bail(fail(GLX_EHIP, "hipSetDevice failed"));
     if (hipStreamCreate(&c->main_s) != hipSuccess || hipStreamCreate(&c->htd_s) != hipSuccess)
         return bail(fail(GLX_EHIP, "hipStreamCreate failed"));
+    for (uint32_t k = 0; k < kMaxShards; k++)
+        if (hipStreamCreate(&c->shard_s[k]) != hipSuccess ||
+            hipEventCreateWithFlags(&c->shard_done[k], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(GLX_EHIP, "shard stream/event creation failed"));
+    c->pb.resize(p->n_planet_buffers < 2 ? 2 : p->n_planet_buffers);
     if ((rc = alloc_set(c, &c->cur)) || (rc = alloc_set(c, &c->tmp))) return bail(rc);
     if (c->keep_rb && ((rc = alloc_rb(c, &c->rb)) || (rc = alloc_rb(c, &c->tmp_rb)))) return bail(rc);
     const size_t tiles = glx::part_tiles(c->cap);
@@ -325,7 +342,8 @@ int glx_get_counts(glx_ctx* c, uint64_t* n_alive, uint64_t* n_total) {
 int glx_upload_planet_block(glx_ctx* c, int buf, uint32_t n_bodies, const double* m, const uint32_t* id,
                             const double* rh, const double* r_log, const double* h0_log, uint32_t n_steps) {
     if (!c) return fail(GLX_EINVAL, "null context");
-    if (buf != 0 && buf != 1) return fail(GLX_EINVAL, "planet buffer %d is not 0 or 1", buf);
+    if (buf < 0 || buf >= (int)c->pb.size())
+        return fail(GLX_EINVAL, "planet buffer %d outside [0, %zu)", buf, c->pb.size());
     if (n_bodies < 1 || n_bodies > c->nb_max)
         return fail(GLX_EINVAL, "n_bodies %u outside [1, %u]", n_bodies, c->nb_max);
     if (n_steps < 1 || n_steps > c->tb_max) return fail(GLX_EINVAL, "n_steps %u outside [1, %u]", n_steps, c->tb_max);
@@ -356,22 +374,28 @@ int glx_upload_planet_block(glx_ctx* c, int buf, uint32_t n_bodies, const double
     return GLX_OK;
 }
 
-int glx_run_block(glx_ctx* c, int buf, double dt, double outer_radius, int form) {
-    if (!c) return fail(GLX_EINVAL, "null context");
-    if (buf != 0 && buf != 1) return fail(GLX_EINVAL, "planet buffer %d is not 0 or 1", buf);
+}  // extern "C"
+
+namespace {
+
+int check_form(int form) {
     if (form != GLX_FORM_FP64 && form != GLX_FORM_FAST && form != GLX_FORM_MIXED)
         return fail(GLX_EINVAL, "kernel form %d is not available", form);
-    PlanetBuf& b = c->pb[buf];
-    if (!b.valid) return fail(GLX_ESTATE, "planet buffer %d holds no block", buf);
-    if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
-    GLX_HIP(hipSetDevice(c->device));
+    return GLX_OK;
+}
+
+// Kernel arguments for one block from planet buffer b; `lds_budget` bytes
+// of planet rows are staged per workgroup.
+BlockArgs block_args(glx_ctx* c, const PlanetBuf& b, double dt, double outer_radius, size_t lds_budget) {
     BlockArgs A = base_args(c);
     A.log = b.dev_log;
+    A.first = 0;
     A.n = (uint32_t)c->n_alive;
+    A.step_offset = 0;
     A.n_steps = b.n_steps;
     A.n_bodies = b.n_bodies;
     const size_t row_bytes = (size_t)b.n_bodies * 3 * sizeof(double);
-    uint32_t chunk = (uint32_t)(kLdsBudget / row_bytes);
+    uint32_t chunk = (uint32_t)(lds_budget / row_bytes);
     if (chunk < 1) chunk = 1;
     if (chunk > b.n_steps) chunk = b.n_steps;
     A.chunk = chunk;
@@ -386,11 +410,85 @@ int glx_run_block(glx_ctx* c, int buf, double dt, double outer_radius, int form)
     A.dominant = 1;
     for (uint32_t j = 2; j < b.n_bodies; j++)
         if (b.pc.m[j] > b.pc.m[A.dominant]) A.dominant = j;
+    return A;
+}
+
+int check_buf(glx_ctx* c, int buf) {
+    if (buf < 0 || buf >= (int)c->pb.size())
+        return fail(GLX_EINVAL, "planet buffer %d outside [0, %zu)", buf, c->pb.size());
+    if (!c->pb[buf].valid) return fail(GLX_ESTATE, "planet buffer %d holds no block", buf);
+    return GLX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glx_run_block(glx_ctx* c, int buf, double dt, double outer_radius, int form) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_buf(c, buf)) return rc;
+    if (int rc = check_form(form)) return rc;
+    if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
+    GLX_HIP(hipSetDevice(c->device));
+    PlanetBuf& b = c->pb[buf];
+    const BlockArgs A = block_args(c, b, dt, outer_radius, kLdsBudget);
     GLX_HIP(hipStreamWaitEvent(c->main_s, b.uploaded, 0));
     GLX_HIP(hipEventRecord(c->k0, c->main_s));
-    GLX_HIP(glx::launch_block(A, form, c->main_s));
+    GLX_HIP(glx::launch_block(A, form, 0, c->main_s));
     GLX_HIP(hipEventRecord(c->k1, c->main_s));
     GLX_HIP(hipEventRecord(b.consumed, c->main_s));
+    c->kernel_timed = true;
+    return GLX_OK;
+}
+
+int glx_run_blocks(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
+                   uint32_t n_shards, uint32_t waves_per_simd) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_form(form)) return rc;
+    if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
+    const int nbuf = (int)c->pb.size();
+    if (n_blocks < 1 || (int)n_blocks > nbuf)
+        return fail(GLX_EINVAL, "n_blocks %u outside [1, %d planet buffers]", n_blocks, nbuf);
+    if (n_shards < 1 || n_shards > kMaxShards) return fail(GLX_EINVAL, "n_shards %u outside [1, %d]", n_shards, kMaxShards);
+    if (waves_per_simd != 0 && (waves_per_simd < 3 || waves_per_simd > 8))
+        return fail(GLX_EINVAL, "waves_per_simd %u: use 0 (no cap) or 3..8", waves_per_simd);
+    for (uint32_t k = 0; k < n_blocks; k++) {
+        const int buf = (first_buf + (int)k) % nbuf;
+        if (int rc = check_buf(c, buf)) return rc;
+        if (c->pb[buf].n_bodies != c->pb[first_buf % nbuf].n_bodies)
+            return fail(GLX_EINVAL, "planet buffers of one launch must hold the same bodies");
+    }
+    GLX_HIP(hipSetDevice(c->device));
+    // One workgroup holds one wave per SIMD, so asking for 1/waves of the
+    // CU's LDS caps the resident waves per SIMD; freed slots then go to
+    // whichever shard's next block is ready.
+    const size_t lds_request = waves_per_simd ? (kLdsPerCu / waves_per_simd) & ~(size_t)1023 : 0;
+    const size_t budget = lds_request ? lds_request : kLdsBudget;
+    const uint32_t n = (uint32_t)c->n_alive;
+    uint32_t per = (n + n_shards - 1) / n_shards;
+    per = (per + 255) / 256 * 256;
+    GLX_HIP(hipEventRecord(c->k0, c->main_s));
+    for (uint32_t s = 0; s < n_shards; s++) GLX_HIP(hipStreamWaitEvent(c->shard_s[s], c->k0, 0));
+    uint32_t offset = 0;
+    for (uint32_t k = 0; k < n_blocks; k++) {
+        PlanetBuf& b = c->pb[(first_buf + (int)k) % nbuf];
+        BlockArgs A = block_args(c, b, dt, outer_radius, budget);
+        A.step_offset = offset;
+        for (uint32_t s = 0; s < n_shards; s++) {
+            A.first = s * per;
+            A.n = std::min<uint32_t>(n, (s + 1) * per);
+            if (A.first >= A.n) continue;
+                    GLX_HIP(hipStreamWaitEvent(c->shard_s[s], b.uploaded, 0));
+            GLX_HIP(glx::launch_block(A, form, lds_request, c->shard_s[s]));
+        }
+        offset += b.n_steps;
+    }
+    for (uint32_t s = 0; s < n_shards; s++) {
+        GLX_HIP(hipEventRecord(c->shard_done[s], c->shard_s[s]));
+        GLX_HIP(hipStreamWaitEvent(c->main_s, c->shard_done[s], 0));
+    }
+    GLX_HIP(hipEventRecord(c->k1, c->main_s));
+    for (uint32_t k = 0; k < n_blocks; k++) GLX_HIP(hipEventRecord(c->pb[(first_buf + (int)k) % nbuf].consumed, c->main_s));
     c->kernel_timed = true;
     return GLX_OK;
 }

@@ -25,7 +25,9 @@ struct RollbackSet {
     uint32_t* id;
 };
 
-hipError_t launch_block(const BlockArgs& A, int form, hipStream_t stream);
+// lds_bytes: dynamic LDS to request per workgroup (at least the staged
+// chunk); a larger request caps the workgroups resident per CU.
+hipError_t launch_block(const BlockArgs& A, int form, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_init_accel(const BlockArgs& A, hipStream_t stream);
 
 // Stable partition scratch sizes for n elements.

@@ -53,7 +53,9 @@ struct BlockArgs {
     uint16_t* flags;
     uint32_t* dti;
     const double* log;
+    uint32_t first;        // particles [first, n) of this launch
     uint32_t n;
+    uint32_t step_offset;  // steps of the launch before this block (death index base)
     uint32_t n_steps;
     uint32_t chunk;     // steps staged in LDS at a time
     uint32_t n_bodies;  // used by the generic (NB == 0) instantiation

@@ -16,7 +16,7 @@ namespace glx {
 template <int NB, int FORM>
 __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
     extern __shared__ double lds[];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = A.first + blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < A.n;
     const uint32_t nb = NB > 0 ? (uint32_t)NB : A.n_bodies;
     const uint32_t row = nb * 3;
@@ -34,6 +34,9 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         p.ay = A.ay[i];
         p.az = A.az[i];
         p.flags = A.flags[i];
+        // particles flagged before this block keep their death step;
+        // the others count steps from the block's offset in the launch
+        dti = p.flags == 0 ? A.step_offset : A.dti[i];
     } else {
         p.rx = p.ry = p.rz = p.vx = p.vy = p.vz = p.ax = p.ay = p.az = 0.0;
         p.flags = 0xFFFFu;  // padding lanes never step
@@ -63,7 +66,7 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
                         accel_fast<NB>(p, rowp, A, nb);
                     kick_fma(p, A.half_dt);
                 }
-                dti = cs + s + 1;
+                dti = A.step_offset + cs + s + 1;
             }
         }
     }
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(256) void init_accel_kernel(BlockArgs A) {
     const uint32_t nb = NB > 0 ? (uint32_t)NB : A.n_bodies;
     for (uint32_t k = threadIdx.x; k < nb * 3; k += blockDim.x) lds[k] = A.log[k];
     __syncthreads();
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = A.first + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
     Particle p;
     p.rx = A.rx[i];
@@ -105,10 +108,9 @@ __global__ __launch_bounds__(256) void init_accel_kernel(BlockArgs A) {
 }
 
 template <int FORM>
-static hipError_t launch_block_form(const BlockArgs& A, hipStream_t stream) {
+static hipError_t launch_block_form(const BlockArgs& A, size_t lds_bytes, hipStream_t stream) {
     const uint32_t threads = 256;
-    const uint32_t blocks = (A.n + threads - 1) / threads;
-    const size_t lds_bytes = (size_t)A.chunk * A.n_bodies * 3 * sizeof(double);
+    const uint32_t blocks = (A.n - A.first + threads - 1) / threads;
     switch (A.n_bodies) {
 #define GLX_CASE(NBV)                                                                          \
     case NBV:                                                                                  \
@@ -130,15 +132,17 @@ static hipError_t launch_block_form(const BlockArgs& A, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_block(const BlockArgs& A, int form, hipStream_t stream) {
-    if (A.n == 0) return hipSuccess;
+hipError_t launch_block(const BlockArgs& A, int form, size_t lds_bytes, hipStream_t stream) {
+    if (A.n <= A.first) return hipSuccess;
+    const size_t need = (size_t)A.chunk * A.n_bodies * 3 * sizeof(double);
+    if (lds_bytes < need) lds_bytes = need;
     switch (form) {
         case GLX_FORM_FP64:
-            return launch_block_form<GLX_FORM_FP64>(A, stream);
+            return launch_block_form<GLX_FORM_FP64>(A, lds_bytes, stream);
         case GLX_FORM_FAST:
-            return launch_block_form<GLX_FORM_FAST>(A, stream);
+            return launch_block_form<GLX_FORM_FAST>(A, lds_bytes, stream);
         case GLX_FORM_MIXED:
-            return launch_block_form<GLX_FORM_MIXED>(A, stream);
+            return launch_block_form<GLX_FORM_MIXED>(A, lds_bytes, stream);
         default:
             return hipErrorInvalidValue;
     }

@@ -39,6 +39,7 @@ class Params(C.Structure):
         ("tbsize_max", C.c_uint32),
         ("device", C.c_int32),
         ("keep_rollback", C.c_int32),
+        ("n_planet_buffers", C.c_uint32),
     ]
 
 
@@ -53,6 +54,8 @@ SIGNATURES = {
     "glx_get_counts": (C.c_int, [C.c_void_p, _u64p, _u64p]),
     "glx_upload_planet_block": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, _dp, _u32p, _dp, _dp, _dp, C.c_uint32]),
     "glx_run_block": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
+    "glx_run_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint32,
+                                 C.c_uint32]),
     "glx_init_accel": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _u32p, _dp, _dp, _dp, C.c_double]),
     "glx_kill_encounters": (C.c_int, [C.c_void_p]),
     "glx_partition": (C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p]),
@@ -111,9 +114,9 @@ class Context:
     """One device context: particle arrays, planet-block buffers, streams."""
 
     def __init__(self, n_particles: int, n_bodies_max: int, tbsize_max: int, device: int = 0,
-                 keep_rollback: bool = False):
+                 keep_rollback: bool = False, n_planet_buffers: int = 2):
         self.lib = load()
-        p = Params(n_particles, n_bodies_max, tbsize_max, device, int(keep_rollback))
+        p = Params(n_particles, n_bodies_max, tbsize_max, device, int(keep_rollback), n_planet_buffers)
         h = C.c_void_p()
         self._check(self.lib.glx_create(C.byref(p), C.byref(h)))
         self.h = h
@@ -184,6 +187,12 @@ class Context:
     def run_block(self, buf: int, dt: float, outer_radius: float, form: str | int = "fp64") -> None:
         f = FORMS[form] if isinstance(form, str) else int(form)
         self._check(self.lib.glx_run_block(self.h, buf, dt, outer_radius, f))
+
+    def run_blocks(self, n_blocks: int, first_buf: int, dt: float, outer_radius: float, form: str | int = "fp64",
+                   n_shards: int = 4, waves_per_simd: int = 3) -> None:
+        f = FORMS[form] if isinstance(form, str) else int(form)
+        self._check(self.lib.glx_run_blocks(self.h, n_blocks, first_buf, dt, outer_radius, f, n_shards,
+                                            waves_per_simd))
 
     def init_accel(self, m, rh, planet_r, h0, outer_radius: float, ids=None) -> None:
         m = np.ascontiguousarray(m, dtype=np.float64)

@@ -58,6 +58,7 @@ typedef struct glx_params {
     uint32_t tbsize_max;   /* Time-Block-Size: rows of one planet log */
     int32_t device;        /* HIP device ordinal */
     int32_t keep_rollback; /* allocate the rollback copy (Resolve-Encounters 1) */
+    uint32_t n_planet_buffers; /* planet-block buffers (0 or < 2 means 2: the reference's double buffer) */
 } glx_params;
 
 /* Create / destroy a context and all its device memory.
@@ -90,7 +91,7 @@ int glx_download_particles(glx_ctx* ctx, uint64_t begin, uint64_t n, double* r, 
 int glx_set_counts(glx_ctx* ctx, uint64_t n_alive, uint64_t n_total);
 int glx_get_counts(glx_ctx* ctx, uint64_t* n_alive, uint64_t* n_total);
 
-/* Upload one planet block into buffer buf (0 or 1), async on the upload
+/* Upload one planet block into buffer buf (0 .. n_planet_buffers-1), async on the upload
  * stream.  m, rh: n_bodies doubles (GM units; m[0] is the Sun); id: n_bodies
  * planet ids; r_log: n_steps x (n_bodies-1) x 3 heliocentric planet
  * positions, row s at t0 + (s+1) dt; h0_log: n_steps x 3 indirect terms.
@@ -105,6 +106,20 @@ int glx_upload_planet_block(glx_ctx* ctx, int buf, uint32_t n_bodies, const doub
  * Replaces WHCudaIntegrator::integrate_particles_timeblock_cuda
  * (src/wh.cu:344-395) and the MVSKernel it launches (src/wh.cu:15-207). */
 int glx_run_block(glx_ctx* ctx, int buf, double dt, double outer_radius, int form);
+
+/* Advance particles [0, n_alive) through n_blocks consecutive planet blocks
+ * held in buffers first_buf, first_buf+1, ... (modulo the buffer count),
+ * async.  The particles are split into n_shards slices, each run on its own
+ * stream as a chain of single-block launches, with no compaction between
+ * blocks (flagged particles simply stop).  waves_per_simd > 0 caps the
+ * resident waves per SIMD (3..8) through the LDS request, so that a SIMD
+ * freed early picks up the next ready block of another slice; 0 leaves
+ * occupancy uncapped.  deathtime_index becomes the step count within the
+ * launch (k * Time-Block-Size + step + 1).  With n_blocks = 1 and
+ * n_shards = 1 it is glx_run_block.  No reference counterpart: the
+ * reference launches one block per host iteration (src/executor.cu:640-646). */
+int glx_run_blocks(glx_ctx* ctx, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
+                   uint32_t n_shards, uint32_t waves_per_simd);
 
 /* Initial accelerations and flags of particles [0, n_alive) from planet row
  * 0 of `planet_r` ((n_bodies-1) x 3) and h0 (3 doubles), async.  The

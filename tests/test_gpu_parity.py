@@ -253,3 +253,48 @@ def test_fast_forms_encounter_flags(native, oracle, form):
     low, high = compare_flags(cpu["flags"], gpu["flags"], cpu["id"], gpu["id"])
     print(f"{form}: flagged={int((cpu['flags'] != 0).sum())} mismatched={len(low)}")
     assert len(low) <= 2 and len(high) <= 2
+
+
+@pytest.mark.parametrize("form", ["fp64", "fast"])
+@pytest.mark.parametrize("shards,cap", [(1, 0), (4, 3), (3, 4), (8, 3)])
+def test_run_blocks_equals_single_blocks(native, oracle, form, shards, cap):
+    """One multi-block launch (sharded, occupancy-capped) == the same blocks
+    launched one by one without compaction in between: bit-identical state
+    and flags; deathtime_index counts steps from the launch start."""
+    from glisser_amd import systems
+    s = systems.three_body(5000, seed=31, a_range=(20.0, 40.0), e_hi=0.6, i_hi_deg=5.0)
+    T, K = 256, 4
+    h0_0, rh0, blocks = planet_blocks(oracle, s, K, T, 30.0, cull_radius=3.0, rh_factor=3.5)
+    results = []
+    for multi in (False, True):
+        with native.Context(s.n, 2, T, n_planet_buffers=K) as ctx:
+            ctx.upload(s.pr, s.pv, np.zeros(s.n, np.uint16), s.ids, a=np.zeros((s.n, 3)),
+                       dti=np.zeros(s.n, np.uint32))
+            ctx.set_counts(s.n, s.n)
+            ctx.init_accel(s.m, rh0, s.r[1:], h0_0, OUTER)
+            ctx.kill_encounters()
+            ctx.partition()
+            n_alive, _ = ctx.counts()
+            for k, (rl, h0, rh) in enumerate(blocks):
+                ctx.upload_planet_block(k, s.m, rh, rl, h0)
+            death = np.full(n_alive, -1)
+            if multi:
+                ctx.run_blocks(K, 0, 30.0, OUTER, form, shards, cap)
+            else:
+                for k in range(K):
+                    ctx.run_block(k, 30.0, OUTER, form)
+                    ctx.sync()
+                    d = ctx.download(n_alive)
+                    newly = (d["flags"] != 0) & (death < 0)
+                    death[newly] = k * T + d["dti"][newly]
+            ctx.sync()
+            d = ctx.download(n_alive)
+            d["death"] = death
+            results.append(d)
+    single, multi = results
+    for key in ("r", "v", "a", "flags", "id"):
+        np.testing.assert_array_equal(multi[key], single[key], err_msg=key)
+    dead = multi["flags"] != 0
+    assert dead.sum() > 50
+    np.testing.assert_array_equal(multi["dti"][dead], single["death"][dead])
+    assert np.all(multi["dti"][~dead] == K * T)
