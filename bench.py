@@ -44,7 +44,28 @@ HEADLINE_FORM = "fast"
 # summed over the block kernel and divided by its particle-steps); see
 # profiles/wh_block_counters_200k.md.  Used with the live rate to report
 # the counter-based share of FP64 vector peak.
-COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": None, "mixed": None}
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 362.0, "mixed": 320.0}
+
+# HBM bytes per time-block of the fast form at this workload, from
+# rocprofv3 FETCH_SIZE (x 2.0, the factor a copy kernel of known size needs)
+# and WRITE_SIZE (x 1.0); see profiles/wh_block_200k_counters.md.
+TRAFFIC_BYTES_PER_BLOCK = {"fast": 38.3e6}
+
+
+def copy_probe(n: int = 1_000_000, repeats: int = 3) -> None:
+    """Calibration kernel for the HBM byte counters: the rollback snapshot
+    copies r, v, flags, deathtime index and id (58 B per particle) of n
+    particles, `repeats` times."""
+    from glisser_amd import native
+
+    with native.Context(n, 2, 8, keep_rollback=True) as ctx:
+        z = np.zeros((n, 3))
+        ctx.upload(z, z, np.zeros(n, np.uint16), np.arange(n, dtype=np.uint32), a=z, dti=np.zeros(n, np.uint32))
+        ctx.set_counts(n, n)
+        for _ in range(repeats):
+            ctx.snapshot_rollback()
+        ctx.sync()
+    print(json.dumps({"copy_probe_bytes_read": 58 * n, "copy_probe_bytes_written": 58 * n, "repeats": repeats}))
 
 
 def flops_per_particle_step(n_planets: int) -> int:
@@ -150,7 +171,12 @@ def main(argv=None):
     ap.add_argument("--waves-per-simd", type=int, default=3, help="resident-wave cap (0: none)")
     ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args(argv)
+    if args.copy_probe:
+        copy_probe()
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -217,7 +243,10 @@ def main(argv=None):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": h["alg_tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": (TRAFFIC_BYTES_PER_BLOCK[head] * args.steps
+                        if head in TRAFFIC_BYTES_PER_BLOCK and args.particles == N_PARTICLES else None),
+            "traffic_note": "HBM bytes per launch of `steps` blocks (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
+            "counter_fp64_frac": h.get("counter_fp64_frac"),
             "flops_per_particle_step": f_alg,
             "hbm_gbs_algorithmic": hbm_bytes_per_block(args.particles, inputs[0].n_bodies, TBSIZE, workgroups)
             / (h["kernel_ms_per_block"] / 1e3) / 1e9,
