@@ -3,17 +3,20 @@
 Workload: 200,000 test particles around the Sun + 4 giant planets (masses
 9.55e-4, 2.86e-4, 4.37e-5, 5.15e-5 Msun at 5.20/9.55/19.2/30.1 au;
 particles a ~ U[60,65] au, e ~ U[0,0.1], i ~ U[0,5 deg]), dt = 180 d,
-Time-Block-Size 2048.  One bench "step" is one time-block.  The timed
-region is one launch of `--steps` consecutive blocks (all planet logs
-resident in HBM, produced beforehand by the host planet integrator), then
-the encounter-kill transform, the step count and the stable partition.
+Time-Block-Size 2048.  The planets come from a history sampled every
+365 d (host Jacobi WH at 36.5 d, every tenth state kept), uploaded once;
+each block's planet log is interpolated on the device (cubic Hermite + h0).
+One bench "step" is one time-block.  The timed region is the interpolation
+of `--steps` planet logs and one launch of `--steps` consecutive blocks,
+then the encounter-kill transform, the step count and the stable partition.
 
 Every arithmetic form is measured on the same inputs: fp64 (the
 reference's arithmetic, bit-for-bit with the CPU path in every basic
 operation), fast (FP64 throughout, restructured) and mixed (fast + FP32
 r^-3 for the lighter planets).  `value` is the fast form's rate (pure FP64,
 the form whose error against the CPU oracle matches the fp64 form's);
-the others are reported beside it.
+the others are reported beside it, with their orbital-element differences
+from the fp64 form at the end of the run.
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build uses one GPU.
 """
@@ -85,43 +88,46 @@ def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int, n_workgrou
     return state + planets
 
 
+HIST_SPACING = 365.0
+HIST_SUBSTEPS = 10
+
+
 def make_inputs(n_particles: int, n_blocks: int, seed: int):
-    from glisser_amd import native, systems
+    """Swarm + planet history covering n_blocks blocks (plus one spacing)."""
+    from glisser_amd import systems
 
     s = systems.outer_solar_system(n_particles, seed=seed)
-    planets = native.Planets(s.m, s.r, s.v, cull_radius=CULL, rh_factor=0.0, outer_radius=OUTER)
-    h0_0 = planets.h0_initial()
-    rh0 = planets.state()[2]
-    blocks = []
-    for _ in range(n_blocks):
-        planets.recalculate_rh()
-        rh = planets.state()[2]
-        rl, _vl, h0 = planets.block(TBSIZE, DT)
-        blocks.append((rl, h0, rh))
-    return s, h0_0, rh0, blocks
+    hist = systems.planet_history(s, n_blocks * TBSIZE * DT + HIST_SPACING, spacing=HIST_SPACING,
+                                  substeps=HIST_SUBSTEPS, cull_radius=CULL)
+    rh = np.full(s.n_bodies, CULL)  # Encounter-RH-Factor 0: every body's radius is the cull radius
+    return s, hist, rh
 
 
 def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_simd: int):
     from glisser_amd import native
 
-    s, h0_0, rh0, blocks = inputs
+    s, hist, rh = inputs
     n = s.n
     ctx = native.Context(n, s.n_bodies, TBSIZE, n_planet_buffers=warmup + steps)
     try:
         ctx.upload(s.pr, s.pv, np.zeros(n, np.uint16), s.ids, a=np.zeros((n, 3)), dti=np.zeros(n, np.uint32))
         ctx.set_counts(n, n)
-        ctx.init_accel(s.m, rh0, s.r[1:], h0_0, OUTER)
+        ctx.upload_history(hist.m, hist.t, hist.r, hist.v)
+        pr0, h00 = ctx.interp_at(np.array([0.0]))
+        ctx.init_accel(s.m, rh, pr0[0], h00[0], OUTER)
         ctx.kill_encounters()
         ctx.partition()
-        for k, (rl, h0, rh) in enumerate(blocks[: warmup + steps]):
-            ctx.upload_planet_block(k, s.m, rh, rl, h0)
         if warmup:
+            for k in range(warmup):
+                ctx.interp_block(k, k * TBSIZE * DT, DT, TBSIZE, rh)
             ctx.run_blocks(warmup, 0, DT, OUTER, form, shards, waves_per_simd)
             ctx.kill_encounters()
             ctx.partition()
         ctx.sync()
         n_alive, _ = ctx.counts()
         t0 = time.perf_counter()
+        for k in range(warmup, warmup + steps):
+            ctx.interp_block(k, k * TBSIZE * DT, DT, TBSIZE, rh)
         ctx.run_blocks(steps, warmup, DT, OUTER, form, shards, waves_per_simd)
         ctx.kill_encounters()
         psteps = ctx.sum_steps(n_alive)
@@ -130,32 +136,65 @@ def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_s
         elapsed = time.perf_counter() - t0
         kernel_ms, part_ms = ctx.last_times()
         n_end, _ = ctx.counts()
+        final = ctx.download(n)
     finally:
         ctx.close()
     return {"elapsed_s": elapsed, "particle_steps": psteps, "kernel_ms": kernel_ms, "partition_ms": part_ms,
-            "n_alive_start": n_alive, "n_alive_end": n_end}
+            "n_alive_start": n_alive, "n_alive_end": n_end, "final": final}
+
+
+def element_differences(mu: float, ref: dict, other: dict) -> dict:
+    """Orbital-element and energy differences of `other` from `ref` over the
+    particles unflagged in both (matched by id)."""
+    from glisser_amd import native
+
+    def by_id(d):
+        order = np.argsort(d["id"])
+        return d["r"][order], d["v"][order], d["flags"][order]
+
+    r0, v0, f0 = by_id(ref)
+    r1, v1, f1 = by_id(other)
+    keep = (f0 == 0) & (f1 == 0)
+    if not keep.any():
+        return {"n": 0}
+    mus = np.full(int(keep.sum()), mu)
+    e0 = native.to_elements(mus, r0[keep], v0[keep])
+    e1 = native.to_elements(mus, r1[keep], v1[keep])
+
+    def energy(r, v):
+        return 0.5 * np.sum(v * v, axis=1) - mu / np.linalg.norm(r, axis=1)
+
+    da = np.abs(e1[:, 0] - e0[:, 0]) / e0[:, 0]
+    de = np.abs(e1[:, 1] - e0[:, 1])
+    di = np.abs(e1[:, 2] - e0[:, 2])
+    den = np.abs(energy(r1[keep], v1[keep]) - energy(r0[keep], v0[keep])) / np.abs(energy(r0[keep], v0[keep]))
+    stat = lambda x: {"max": float(x.max()), "median": float(np.median(x))}  # noqa: E731
+    return {"n": int(keep.sum()), "da_over_a": stat(da), "de": stat(de), "di_rad": stat(di),
+            "denergy_rel": stat(den), "flags_differ": int(np.sum(f0 != f1))}
 
 
 def cpu_baseline(inputs, sample_particles: int):
     """Reference CPU path (oracle/_ref: the reference's own src/wh.cpp built
     from its sources) or, where that library is absent, the oracle
     restatement: one 2048-step block over the first `sample_particles`
-    particles of the same swarm, single thread."""
+    particles of the same swarm and planet log, single thread."""
     from oracle.pyoracle import Oracle, reference_available
 
-    s, h0_0, rh0, blocks = inputs
+    s, hist, rh = inputs
     kind = "reference" if reference_available() else "port"
+    checker = Oracle()
+    rl, h0 = checker.hermite_log(hist.m, hist.t, hist.r, hist.v, 0.0, DT, TBSIZE)
+    pr0, h00 = checker.hermite_log(hist.m, hist.t, hist.r, hist.v, -DT, DT, 1)
     o = Oracle(reference=(kind == "reference"))
     n = sample_particles
-    a0, f0 = o.particle_accels(s.m, rh0, OUTER, s.r[1:], h0_0, s.pr[:n], np.zeros(n), s.ids[:n])
-    rl, h0, rh = blocks[0]
+    a0, f0 = o.particle_accels(s.m, rh, OUTER, pr0[0], h00[0], s.pr[:n], np.zeros(n), s.ids[:n])
     t = time.perf_counter()
     out = o.particle_block(s.m, rh, OUTER, rl, h0, 0.0, DT, s.pr[:n], s.pv[:n], a0, f0, s.ids[:n])
     el = time.perf_counter() - t
     steps = int(out["dti"].astype(np.int64).sum())
     return {"value": steps / el, "unit": "particle-steps/s", "cores": 1, "kind": kind,
-            "sample": f"{n} particles x 1 block of {TBSIZE} steps of the same swarm ({steps} particle-steps, "
-                      f"{el:.1f} s, single thread)"}
+            "sample": f"{n} particles x 1 block of {TBSIZE} steps of the same swarm and planet log "
+                      f"({steps} particle-steps, {el:.1f} s, single thread)"}
 
 
 def main(argv=None):
@@ -188,11 +227,13 @@ def main(argv=None):
     forms = ["fp64", "fast", "mixed"] if args.form == "all" else [args.form]
     head = HEADLINE_FORM if HEADLINE_FORM in forms else forms[0]
     n_planets = inputs[0].n_bodies - 1
+    finals = {}
     f_alg = flops_per_particle_step(n_planets)
     workgroups = (args.particles + 255) // 256
     per_form = {}
     for form in forms:
         r = run_gpu(inputs, form, args.steps, args.warmup, args.shards, args.waves_per_simd)
+        finals[form] = r.pop("final")
         block_s = r["kernel_ms"] / 1e3 / args.steps
         per_block = r["particle_steps"] / args.steps
         entry = {
@@ -232,7 +273,7 @@ def main(argv=None):
             "blocks_per_launch": args.steps,
             "shards": args.shards,
             "waves_per_simd": args.waves_per_simd,
-            "planet_log": "host Jacobi-WH producer, all blocks resident",
+            "planet_log": f"device cubic-Hermite interpolation of a {HIST_SPACING:g}-d history",
             "seed": args.seed,
             "parallelism": "single-gpu",
         },
@@ -256,6 +297,10 @@ def main(argv=None):
         line["mixed_vs_fp64"] = per_form["mixed"]["value"] / per_form["fp64"]["value"]
     if "fp64" in per_form and "fast" in per_form:
         line["fast_vs_fp64"] = per_form["fast"]["value"] / per_form["fp64"]["value"]
+    if "fp64" in finals:
+        line["precision_vs_fp64"] = {
+            "after_steps": (args.warmup + args.steps) * TBSIZE,
+            **{f: element_differences(inputs[0].m[0], finals["fp64"], finals[f]) for f in finals if f != "fp64"}}
     if rank == 0 and not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs, args.cpu_sample)
     if rank == 0:

@@ -64,6 +64,22 @@ struct PlanetBuf {
     bool valid = false;
 };
 
+constexpr uint32_t kQueryMax = 1024;  // times per glx_interp_at batch
+
+// Sampled planet history resident on the device (glx_upload_history).
+struct History {
+    double* t = nullptr;
+    double* r = nullptr;
+    double* v = nullptr;
+    double* q_times = nullptr;  // [kQueryMax]
+    double* q_rows = nullptr;   // [kQueryMax][GLX_MAX_BODIES][3]
+    uint64_t n_samples = 0;
+    uint32_t n_bodies = 0;
+    double m[GLX_MAX_BODIES] = {};
+    uint32_t id[GLX_MAX_BODIES] = {};
+    bool valid = false;
+};
+
 }  // namespace
 
 namespace glx {
@@ -94,6 +110,8 @@ struct glx_ctx {
     hipEvent_t k0 = nullptr, k1 = nullptr, q0 = nullptr, q1 = nullptr;
     bool kernel_timed = false, part_timed = false;
     uint64_t n_alive = 0, n_total = 0;
+    History hist;
+    double hist_t0 = 0, hist_t1 = 0;  // first and last history times (host copy)
     std::vector<void*> allocs;
 };
 
@@ -133,6 +151,8 @@ void release(glx_ctx* c) {
     for (uint32_t k = 0; k < kMaxShards; k++)
         if (c->shard_s[k]) (void)hipStreamSynchronize(c->shard_s[k]);
     for (void* p : c->allocs) (void)hipFree(p);
+    for (double* p : {c->hist.t, c->hist.r, c->hist.v, c->hist.q_times, c->hist.q_rows})
+        if (p) (void)hipFree(p);
     for (auto& b : c->pb) {
         if (b.pin_log) (void)hipHostFree(b.pin_log);
         if (b.uploaded) (void)hipEventDestroy(b.uploaded);
@@ -478,7 +498,7 @@ int glx_run_blocks(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, doub
             A.first = s * per;
             A.n = std::min<uint32_t>(n, (s + 1) * per);
             if (A.first >= A.n) continue;
-                    GLX_HIP(hipStreamWaitEvent(c->shard_s[s], b.uploaded, 0));
+            GLX_HIP(hipStreamWaitEvent(c->shard_s[s], b.uploaded, 0));
             GLX_HIP(glx::launch_block(A, form, lds_request, c->shard_s[s]));
         }
         offset += b.n_steps;
@@ -490,6 +510,152 @@ int glx_run_blocks(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, doub
     GLX_HIP(hipEventRecord(c->k1, c->main_s));
     for (uint32_t k = 0; k < n_blocks; k++) GLX_HIP(hipEventRecord(c->pb[(first_buf + (int)k) % nbuf].consumed, c->main_s));
     c->kernel_timed = true;
+    return GLX_OK;
+}
+
+int glx_upload_history(glx_ctx* c, uint32_t n_bodies, const double* m, const uint32_t* id, uint64_t n_samples,
+                       const double* t, const double* r, const double* v) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (n_bodies < 1 || n_bodies > c->nb_max)
+        return fail(GLX_EINVAL, "n_bodies %u outside [1, %u]", n_bodies, c->nb_max);
+    if (n_samples < 2) return fail(GLX_EINVAL, "a history needs at least 2 samples");
+    if (!m || !t || (n_bodies > 1 && (!r || !v))) return fail(GLX_EINVAL, "null history array");
+    if (!(m[0] > 0.0)) return fail(GLX_EINVAL, "central mass m[0] must be positive");
+    for (uint64_t k = 0; k + 1 < n_samples; k++)
+        if (!(t[k + 1] > t[k]))
+            return fail(GLX_EINVAL, "history times must increase strictly (sample %llu)", (unsigned long long)k + 1);
+    GLX_HIP(hipSetDevice(c->device));
+    // Kernels queued earlier may still read the old history.
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->htd_s));
+    History& H = c->hist;
+    H.valid = false;
+    for (double** p : {&H.t, &H.r, &H.v}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    const size_t pv = (size_t)n_samples * (n_bodies - 1) * 3;
+    const size_t pv_alloc = pv ? pv : 1;
+    if (hipMalloc((void**)&H.t, n_samples * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&H.r, pv_alloc * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&H.v, pv_alloc * sizeof(double)) != hipSuccess)
+        return fail(GLX_ENOMEM, "hipMalloc of a %llu-sample history failed", (unsigned long long)n_samples);
+    if (!H.q_times) {
+        if (hipMalloc((void**)&H.q_times, kQueryMax * sizeof(double)) != hipSuccess ||
+            hipMalloc((void**)&H.q_rows, (size_t)kQueryMax * GLX_MAX_BODIES * 3 * sizeof(double)) != hipSuccess)
+            return fail(GLX_ENOMEM, "hipMalloc of the interpolation query space failed");
+    }
+    GLX_HIP(hipMemcpy(H.t, t, n_samples * sizeof(double), hipMemcpyHostToDevice));
+    if (pv) {
+        GLX_HIP(hipMemcpy(H.r, r, pv * sizeof(double), hipMemcpyHostToDevice));
+        GLX_HIP(hipMemcpy(H.v, v, pv * sizeof(double), hipMemcpyHostToDevice));
+    }
+    H.n_samples = n_samples;
+    H.n_bodies = n_bodies;
+    std::memset(H.m, 0, sizeof H.m);
+    for (uint32_t j = 0; j < n_bodies; j++) {
+        H.m[j] = m[j];
+        H.id[j] = id ? id[j] : j;
+    }
+    H.valid = true;
+    c->hist_t0 = t[0];
+    c->hist_t1 = t[n_samples - 1];
+    return GLX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+glx::HistoryView history_view(const glx_ctx* c) {
+    glx::HistoryView V{};
+    V.t = c->hist.t;
+    V.r = c->hist.r;
+    V.v = c->hist.v;
+    V.n_samples = c->hist.n_samples;
+    V.n_bodies = c->hist.n_bodies;
+    std::memcpy(V.m, c->hist.m, sizeof V.m);
+    return V;
+}
+
+// Tolerance on leaving the sampled range, as the reference's interpolator
+// allows (src/interp.cpp:131).
+constexpr double kHistSlack = 1e-8;
+
+}  // namespace
+
+extern "C" {
+
+int glx_interp_block(glx_ctx* c, int buf, double t_start, double dt, uint32_t n_steps, const double* rh) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (!c->hist.valid) return fail(GLX_ESTATE, "no planet history uploaded");
+    if (buf < 0 || buf >= (int)c->pb.size())
+        return fail(GLX_EINVAL, "planet buffer %d outside [0, %zu)", buf, c->pb.size());
+    if (n_steps < 1 || n_steps > c->tb_max) return fail(GLX_EINVAL, "n_steps %u outside [1, %u]", n_steps, c->tb_max);
+    if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
+    if (!rh) return fail(GLX_EINVAL, "null rh");
+    const double first = t_start + dt, last = t_start + (double)n_steps * dt;
+    if (!(first >= c->hist_t0 - kHistSlack) || !(last <= c->hist_t1 + kHistSlack))
+        return fail(GLX_EINVAL, "block times [%.17g, %.17g] leave the history [%.17g, %.17g]", first, last,
+                    c->hist_t0, c->hist_t1);
+    GLX_HIP(hipSetDevice(c->device));
+    PlanetBuf& b = c->pb[buf];
+    GLX_HIP(hipStreamWaitEvent(c->htd_s, b.consumed, 0));
+    GLX_HIP(glx::launch_hermite_rows(history_view(c), nullptr, t_start, dt, n_steps, b.dev_log, c->htd_s));
+    GLX_HIP(hipEventRecord(b.uploaded, c->htd_s));
+    fill_consts(c->hist.n_bodies, c->hist.m, c->hist.id, rh, &b.pc);
+    b.mu = c->hist.m[0];
+    b.n_bodies = c->hist.n_bodies;
+    b.n_steps = n_steps;
+    b.valid = true;
+    return GLX_OK;
+}
+
+int glx_interp_at(glx_ctx* c, uint32_t n_times, const double* times, double* r, double* h0) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (!c->hist.valid) return fail(GLX_ESTATE, "no planet history uploaded");
+    if (n_times == 0) return GLX_OK;
+    if (!times) return fail(GLX_EINVAL, "null times");
+    for (uint32_t k = 0; k < n_times; k++)
+        if (!(times[k] >= c->hist_t0 - kHistSlack) || !(times[k] <= c->hist_t1 + kHistSlack))
+            return fail(GLX_EINVAL, "time %.17g leaves the history [%.17g, %.17g]", times[k], c->hist_t0, c->hist_t1);
+    GLX_HIP(hipSetDevice(c->device));
+    const uint32_t nb = c->hist.n_bodies, np = nb - 1;
+    std::vector<double> rows((size_t)kQueryMax * nb * 3);
+    for (uint32_t k0 = 0; k0 < n_times; k0 += kQueryMax) {
+        const uint32_t cnt = std::min(kQueryMax, n_times - k0);
+        GLX_HIP(hipMemcpyAsync(c->hist.q_times, times + k0, cnt * sizeof(double), hipMemcpyHostToDevice, c->htd_s));
+        GLX_HIP(glx::launch_hermite_rows(history_view(c), c->hist.q_times, 0.0, 0.0, cnt, c->hist.q_rows, c->htd_s));
+        GLX_HIP(hipMemcpyAsync(rows.data(), c->hist.q_rows, (size_t)cnt * nb * 3 * sizeof(double),
+                               hipMemcpyDeviceToHost, c->htd_s));
+        GLX_HIP(hipStreamSynchronize(c->htd_s));
+        for (uint32_t k = 0; k < cnt; k++) {
+            const double* row = rows.data() + (size_t)k * nb * 3;
+            if (h0) std::memcpy(h0 + (size_t)(k0 + k) * 3, row, 3 * sizeof(double));
+            if (r && np) std::memcpy(r + (size_t)(k0 + k) * np * 3, row + 3, (size_t)np * 3 * sizeof(double));
+        }
+    }
+    return GLX_OK;
+}
+
+int glx_download_planet_block(glx_ctx* c, int buf, uint32_t n_bodies, uint32_t n_steps, double* r_log,
+                              double* h0_log) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_buf(c, buf)) return rc;
+    const PlanetBuf& b = c->pb[buf];
+    if (n_bodies != b.n_bodies || n_steps != b.n_steps)
+        return fail(GLX_EINVAL, "buffer %d holds %u bodies x %u steps, not %u x %u", buf, b.n_bodies, b.n_steps,
+                    n_bodies, n_steps);
+    GLX_HIP(hipSetDevice(c->device));
+    GLX_HIP(hipEventSynchronize(b.uploaded));
+    const uint32_t nb = b.n_bodies, np = nb - 1;
+    std::vector<double> rows((size_t)b.n_steps * nb * 3);
+    GLX_HIP(hipMemcpy(rows.data(), b.dev_log, rows.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < b.n_steps; s++) {
+        const double* row = rows.data() + (size_t)s * nb * 3;
+        if (h0_log) std::memcpy(h0_log + (size_t)s * 3, row, 3 * sizeof(double));
+        if (r_log && np) std::memcpy(r_log + (size_t)s * np * 3, row + 3, (size_t)np * 3 * sizeof(double));
+    }
     return GLX_OK;
 }
 

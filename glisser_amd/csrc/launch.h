@@ -44,6 +44,23 @@ hipError_t launch_kill_encounters(uint16_t* flags, uint32_t n, hipStream_t strea
 hipError_t launch_snapshot(ParticleSet cur, RollbackSet rb, uint32_t n, hipStream_t stream);
 hipError_t launch_sum_steps(const uint32_t* dti, uint32_t n, unsigned long long* out, hipStream_t stream);
 
+// A sampled planet history on the device: times t[n_samples] (strictly
+// increasing) and heliocentric r, v of bodies 1..n_bodies-1 as
+// [n_samples][n_bodies-1][3]; m holds the bodies' GM (m[0] = Sun).
+struct HistoryView {
+    const double* t;
+    const double* r;
+    const double* v;
+    uint64_t n_samples;
+    uint32_t n_bodies;
+    double m[GLX_MAX_BODIES];
+};
+
+// Planet-log rows (h0, then planet positions; n_bodies x 3 doubles each) at
+// times[s], or at t_start + (s+1) dt when times is null, for s < n_steps.
+hipError_t launch_hermite_rows(const HistoryView& H, const double* times, double t_start, double dt, uint32_t n_steps,
+                               double* out, hipStream_t stream);
+
 // AoS (x,y,z) <-> SoA transposes for host copies staged through device memory.
 hipError_t launch_aos_to_soa(const double* aos, double* x, double* y, double* z, uint32_t n, hipStream_t stream);
 hipError_t launch_soa_to_aos(const double* x, const double* y, const double* z, double* aos, uint32_t n,

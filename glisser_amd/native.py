@@ -53,6 +53,10 @@ SIGNATURES = {
     "glx_set_counts": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     "glx_get_counts": (C.c_int, [C.c_void_p, _u64p, _u64p]),
     "glx_upload_planet_block": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, _dp, _u32p, _dp, _dp, _dp, C.c_uint32]),
+    "glx_upload_history": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _u32p, C.c_uint64, _dp, _dp, _dp]),
+    "glx_interp_block": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_uint32, _dp]),
+    "glx_interp_at": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _dp, _dp]),
+    "glx_download_planet_block": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, _dp, _dp]),
     "glx_run_block": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
     "glx_run_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint32,
                                  C.c_uint32]),
@@ -183,6 +187,39 @@ class Context:
         ids = np.arange(nb, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
         self._check(self.lib.glx_upload_planet_block(self.h, buf, nb, _ptr(m, _dp), _ptr(ids, _u32p),
                                                      _ptr(rh, _dp), _ptr(rl, _dp), _ptr(h0, _dp), steps))
+
+    def upload_history(self, m, t, r, v, ids=None) -> None:
+        """Sampled planet history: times t (k,), heliocentric r, v of the
+        planets (k, n_bodies-1, 3)."""
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        nb = len(m)
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        k = len(t)
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(k, nb - 1, 3)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(k, nb - 1, 3)
+        ids = np.arange(nb, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+        self._hist_bodies = nb
+        self._check(self.lib.glx_upload_history(self.h, nb, _ptr(m, _dp), _ptr(ids, _u32p), k, _ptr(t, _dp),
+                                                _ptr(r, _dp), _ptr(v, _dp)))
+
+    def interp_block(self, buf: int, t_start: float, dt: float, n_steps: int, rh) -> None:
+        rh = np.ascontiguousarray(rh, dtype=np.float64)
+        self._check(self.lib.glx_interp_block(self.h, buf, t_start, dt, n_steps, _ptr(rh, _dp)))
+
+    def interp_at(self, times):
+        """(r (n, n_bodies-1, 3), h0 (n, 3)) interpolated at the given times."""
+        times = np.ascontiguousarray(np.atleast_1d(times), dtype=np.float64)
+        nb = self._hist_bodies
+        r = np.empty((len(times), nb - 1, 3))
+        h0 = np.empty((len(times), 3))
+        self._check(self.lib.glx_interp_at(self.h, len(times), _ptr(times, _dp), _ptr(r, _dp), _ptr(h0, _dp)))
+        return r, h0
+
+    def download_planet_block(self, buf: int, n_bodies: int, n_steps: int):
+        r = np.empty((n_steps, n_bodies - 1, 3))
+        h0 = np.empty((n_steps, 3))
+        self._check(self.lib.glx_download_planet_block(self.h, buf, n_bodies, n_steps, _ptr(r, _dp), _ptr(h0, _dp)))
+        return r, h0
 
     def run_block(self, buf: int, dt: float, outer_radius: float, form: str | int = "fp64") -> None:
         f = FORMS[form] if isinstance(form, str) else int(form)

@@ -91,3 +91,46 @@ def three_body(n_particles: int, seed: int = 4, a_pl=30.0, mu_pl=1e-3, a_range=(
     ppr, ppv = native.from_elements(np.full(n_particles, m[0]), el, mean_anomaly=True)
     return System(m, r, v, ppr, ppv, np.arange(n_particles, dtype=np.uint32), seed,
                   {"kind": "three_body", "a_pl": a_pl, "mu_pl": mu_pl})
+
+
+@dataclass
+class History:
+    """Sampled planet history: times t (k,), heliocentric planet r, v
+    (k, n_bodies - 1, 3), body masses m (n_bodies,)."""
+    m: np.ndarray
+    t: np.ndarray
+    r: np.ndarray
+    v: np.ndarray
+
+    @property
+    def n_bodies(self) -> int:
+        return len(self.m)
+
+
+def planet_history(system: System, t_end: float, spacing: float = 365.0, substeps: int = 10,
+                   cull_radius: float = 0.5) -> History:
+    """Integrate the planets from t = 0 with the host Jacobi WH producer at
+    dt = spacing / substeps and keep every substeps-th state: samples at
+    t = 0, spacing, 2 spacing, ... up to the first sample >= t_end."""
+    n_samples = int(np.ceil(t_end / spacing)) + 1
+    pl = native.Planets(system.m, system.r, system.v, cull_radius=cull_radius)
+    try:
+        npl = system.n_bodies - 1
+        r = np.empty((n_samples, npl, 3))
+        v = np.empty((n_samples, npl, 3))
+        r[0] = system.r[1:]
+        v[0] = system.v[1:]
+        dt = spacing / substeps
+        chunk = max(1, 4096 // substeps) * substeps
+        k = 1
+        while k < n_samples:
+            steps = min(chunk, (n_samples - k) * substeps)
+            rl, vl, _ = pl.block(steps, dt)
+            got = steps // substeps
+            r[k:k + got] = rl[substeps - 1::substeps]
+            v[k:k + got] = vl[substeps - 1::substeps]
+            k += got
+    finally:
+        pl.close()
+    t = np.arange(n_samples, dtype=np.float64) * spacing
+    return History(system.m.copy(), t, r, v)

@@ -101,6 +101,34 @@ int glx_get_counts(glx_ctx* ctx, uint64_t* n_alive, uint64_t* n_total);
 int glx_upload_planet_block(glx_ctx* ctx, int buf, uint32_t n_bodies, const double* m, const uint32_t* id,
                             const double* rh, const double* r_log, const double* h0_log, uint32_t n_steps);
 
+/* Upload a sampled planet history, synchronous: n_samples >= 2 times t
+ * (strictly increasing) with heliocentric r and v of bodies 1..n_bodies-1
+ * (n_samples x (n_bodies-1) x 3 each); m: n_bodies GM values (m[0] is the
+ * Sun); id: n_bodies planet ids (NULL means 0..n_bodies-1).  Replaces the
+ * history reader of the Interpolator (src/interp.cpp:14-96); the history
+ * here is Cartesian and interpolated with cubic Hermite polynomials. */
+int glx_upload_history(glx_ctx* ctx, uint32_t n_bodies, const double* m, const uint32_t* id, uint64_t n_samples,
+                       const double* t, const double* r, const double* v);
+
+/* Fill planet buffer buf with n_steps rows interpolated from the history at
+ * t_start + (s+1) dt, async on the upload stream: no host work and no copy.
+ * rh: n_bodies encounter radii.  GLX_EINVAL if a time leaves the history
+ * (by more than 1e-8).  Replaces Interpolator::fill, WHIntegrator::load_h0
+ * and the planet-log upload (src/interp.cpp:113-162, src/wh.cpp:993-1025,
+ * src/executor.cu:442-461). */
+int glx_interp_block(glx_ctx* ctx, int buf, double t_start, double dt, uint32_t n_steps, const double* rh);
+
+/* Interpolated planet positions (n_times x (n_bodies-1) x 3) and h0
+ * (n_times x 3) at arbitrary times, synchronous; either output may be NULL.
+ * Interpolator::fill_one (src/interp.cpp:99-111) plus load_h0. */
+int glx_interp_at(glx_ctx* ctx, uint32_t n_times, const double* times, double* r, double* h0);
+
+/* Copy the rows of planet buffer buf back, synchronous: r_log
+ * n_steps x (n_bodies-1) x 3 and h0_log n_steps x 3 (either may be NULL);
+ * n_bodies and n_steps must be those of the block the buffer holds. */
+int glx_download_planet_block(glx_ctx* ctx, int buf, uint32_t n_bodies, uint32_t n_steps, double* r_log,
+                              double* h0_log);
+
 /* Advance particles [0, n_alive) through every step of planet buffer buf
  * (async on the main stream; waits for that buffer's upload).
  * Replaces WHCudaIntegrator::integrate_particles_timeblock_cuda

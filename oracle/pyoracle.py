@@ -39,7 +39,11 @@ _SIGS = {
     "to_elements": (None, [C.c_double, _dp, _dp, _dp]),
     "from_elements": (None, [C.c_double, _dp, _dp, _dp]),
 }
-_ORACLE_ONLY = {"drift": (C.c_int, [C.c_double, C.c_uint32, _dp, _dp, _dp, C.c_int, _u64p])}
+_ORACLE_ONLY = {
+    "drift": (C.c_int, [C.c_double, C.c_uint32, _dp, _dp, _dp, C.c_int, _u64p]),
+    "hermite_log": (C.c_int, [C.c_uint32, _dp, C.c_uint64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_uint32,
+                              _dp, _dp]),
+}
 
 
 def _p(a, t):
@@ -126,6 +130,22 @@ class Oracle:
         bad = C.c_uint64(0)
         rc = self._f("drift")(t, n, _p(r, _dp), _p(v, _dp), _p(mu, _dp), int(fixed), C.byref(bad))
         return rc, r, v
+
+    def hermite_log(self, m, t_samples, r_samples, v_samples, t_start, dt, n_steps):
+        """Cubic Hermite planet log from a sampled history: (r_log, h0_log)
+        for times t_start + (s+1) dt, s < n_steps."""
+        m = np.ascontiguousarray(m, np.float64)
+        ts = np.ascontiguousarray(t_samples, np.float64)
+        rs = np.ascontiguousarray(r_samples, np.float64)
+        vs = np.ascontiguousarray(v_samples, np.float64)
+        nb = len(m)
+        rl = np.empty((n_steps, nb - 1, 3))
+        h0 = np.empty((n_steps, 3))
+        rc = self._f("hermite_log")(nb, _p(m, _dp), len(ts), _p(ts, _dp), _p(rs, _dp), _p(vs, _dp), t_start, dt,
+                                    n_steps, _p(rl, _dp), _p(h0, _dp))
+        if rc != 0:
+            raise OracleError(rc, self.err())
+        return rl, h0
 
     # -- planets ----------------------------------------------------------
     def planets(self, m, r, v, ids=None, cull_radius=0.5, rh_factor=0.0, outer=300.0):

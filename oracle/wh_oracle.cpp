@@ -21,6 +21,8 @@
 //   planet step / h0           : golden vectors, energy/L conservation
 //   stable_partition           : std::stable_partition property tests
 //   to_elements/from_elements  : round trip, circular/planar special cases
+//   hermite_log                : exact for cubics, node reproduction,
+//                                fourth-order convergence on Kepler orbits
 //
 // Error behaviour: where the reference throws std::runtime_error (unbound
 // orbit, unconverged Kepler solve; src/wh.cpp:894-908,942-945) the oracle
@@ -115,7 +117,7 @@ bool kepler_var(double dM, double esinEo, double ecosEo, double& dE, double& sin
     return true;
 }
 
-enum Status : int { kOk = 0, kUnbound = 1, kUnconverged = 2 };
+enum Status : int { kOk = 0, kUnbound = 1, kUnconverged = 2, kBadInput = 3 };
 
 // Heliocentric Kepler drift of entries [start, start+n) by time t about the
 // per-entry central mass mu[i]; masked entries keep r and v.  Restates
@@ -629,6 +631,61 @@ void orc_load_h0(uint32_t n_bodies, const double* m, const double* r_log, uint32
         add_to(h0, a_common);
         store3(h0_log, s, h0);
     }
+}
+
+// Cubic Hermite interpolation of a sampled planet history into a planet log
+// (no reference counterpart: the reference interpolates orbital elements
+// linearly, src/interp.cpp:113-293; cubic Hermite in Cartesian coordinates
+// is this project's history format).  Samples k = 0..n_samples-1 at strictly
+// increasing times t[k] hold heliocentric r and v of bodies 1..n_bodies-1
+// ([k][planet][3] each).  Row s of the output is the state at
+// t_start + (s+1) dt -- the planet-log convention of src/wh.cpp:1060 -- and
+// h0_log row s is the load_h0 indirect term (src/wh.cpp:993-1025) of that
+// row.  On the interval [t_k, t_k+1] with h = t_k+1 - t_k and
+// u = (t - t_k) / h:
+//   r(t) = h00 r_k + h10 h v_k + h01 r_k+1 + h11 h v_k+1,
+//   h00 = 2u^3 - 3u^2 + 1, h10 = u^3 - 2u^2 + u, h01 = 3u^2 - 2u^3,
+//   h11 = u^3 - u^2.
+// k is the last sample with t_k <= t, clamped to [0, n_samples-2]; the
+// evaluation order written here is the one the device kernel uses, so the
+// two agree bit for bit.  Pinned by: exactness for cubic trajectories, node
+// reproduction, fourth-order convergence on Kepler orbits (tests).
+int orc_hermite_log(uint32_t n_bodies, const double* m, uint64_t n_samples, const double* t_s, const double* r_s,
+                    const double* v_s, double t_start, double dt, uint32_t n_steps, double* r_log, double* h0_log) {
+    if (n_samples < 2 || n_bodies < 1) {
+        std::snprintf(g_err, sizeof g_err, "hermite: need >= 2 samples and >= 1 body");
+        return kBadInput;
+    }
+    const uint32_t np = n_bodies - 1;
+    for (uint32_t s = 0; s < n_steps; s++) {
+        const double t = t_start + (double)(s + 1) * dt;
+        uint64_t lo = 0, hi = n_samples - 1;
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            if (t_s[mid] <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const double h = t_s[lo + 1] - t_s[lo];
+        const double u = (t - t_s[lo]) / h;
+        const double u2 = u * u, u3 = u2 * u;
+        const double h00 = 2.0 * u3 - 3.0 * u2 + 1.0;
+        const double h10 = u3 - 2.0 * u2 + u;
+        const double h01 = 3.0 * u2 - 2.0 * u3;
+        const double h11 = u3 - u2;
+        const double w10 = h10 * h, w11 = h11 * h;
+        for (uint32_t j = 0; j < np; j++) {
+            const double* r0 = r_s + (lo * np + j) * 3;
+            const double* v0 = v_s + (lo * np + j) * 3;
+            const double* r1 = r0 + (size_t)np * 3;
+            const double* v1 = v0 + (size_t)np * 3;
+            double* out = r_log + ((size_t)s * np + j) * 3;
+            for (int c = 0; c < 3; c++) out[c] = h00 * r0[c] + w10 * v0[c] + h01 * r1[c] + w11 * v1[c];
+        }
+        if (h0_log) orc_load_h0(n_bodies, m, r_log + (size_t)s * np * 3, 1, h0_log + (size_t)s * 3);
+    }
+    return kOk;
 }
 
 // calculate_planet_metrics (src/wh.cpp:422-470): barycentric energy and

@@ -303,3 +303,82 @@ def test_elements_round_trip(oracle):
     r, v = oracle.from_elements(MU, np.array([10.0, 0.0, 0.0, 0.0, 0.0, 0.7]))
     back = oracle.to_elements(MU, r, v)
     assert back[1] < 1e-13 and back[2] == 0.0 and abs(back[5] - 0.7) < 1e-12
+
+
+# ----------------------------------------------------- planet history ----
+
+def _cubic_history(rng, t, n_planets):
+    """Planet tracks that are exact cubics in t: r(t) = c0 + c1 t + c2 t^2 +
+    c3 t^3 per component, v = r'(t)."""
+    c = rng.normal(size=(4, n_planets, 3)) * np.array([10.0, 1e-2, 1e-5, 1e-8])[:, None, None]
+
+    def r_of(tt):
+        tt = np.asarray(tt)[..., None, None]
+        return c[0] + c[1] * tt + c[2] * tt**2 + c[3] * tt**3
+
+    def v_of(tt):
+        tt = np.asarray(tt)[..., None, None]
+        return c[1] + 2 * c[2] * tt + 3 * c[3] * tt**2
+
+    return r_of, v_of
+
+
+def test_hermite_exact_for_cubics(oracle):
+    """Cubic Hermite reproduces any cubic trajectory (to rounding): a
+    dropped basis term, a missing h factor on the velocities or swapped
+    end points all fail this."""
+    rng = np.random.default_rng(51)
+    t = np.cumsum(rng.uniform(200.0, 500.0, 40))  # irregular sample times
+    m = np.array([MU, 1e-6, 2e-6, 3e-7])
+    r_of, v_of = _cubic_history(rng, t, 3)
+    dt = 37.0
+    t_start = t[0] - dt + 0.5
+    n_steps = int((t[-1] - t[0] - 1.0) / dt)
+    rl, h0 = oracle.hermite_log(m, t, r_of(t), v_of(t), t_start, dt, n_steps)
+    times = t_start + (np.arange(n_steps) + 1.0) * dt
+    exact = r_of(times)
+    scale = np.abs(exact).max()
+    assert np.abs(rl - exact).max() / scale < 1e-13
+    # h0 is load_h0 of the interpolated rows, bit for bit
+    np.testing.assert_array_equal(h0, oracle.load_h0(m, rl))
+
+
+def test_hermite_reproduces_nodes(oracle):
+    """Rows that fall on a sample time return the sample exactly (u = 0)."""
+    from glisser_amd import systems
+    s = systems.outer_solar_system(1, seed=3)
+    h = systems.planet_history(s, 20 * 365.0, spacing=365.0)
+    dt = 365.0 / 4  # 91.25: exact in binary, so t_start + (s+1) dt hits the nodes
+    rl, _ = oracle.hermite_log(h.m, h.t, h.r, h.v, -dt, dt, 80)
+    for k in range(20):  # row s is at t = s dt
+        np.testing.assert_array_equal(rl[4 * k], h.r[k])
+
+
+def test_hermite_fourth_order_on_kepler_orbits(oracle):
+    """On real orbits (sampled with the pinned Kepler drift) the error at
+    interval midpoints falls ~16x when the spacing halves."""
+    from glisser_amd import systems
+    s = systems.outer_solar_system(1, seed=5)
+    r0, v0 = s.r[1:], s.v[1:]
+    mu = s.m[0] + s.m[1:]
+    errs = []
+    for spacing in (200.0, 100.0, 50.0):
+        t = np.arange(0.0, 4001.0, spacing)
+        rs = np.empty((len(t), 4, 3))
+        vs = np.empty((len(t), 4, 3))
+        for k, tk in enumerate(t):
+            _, rs[k], vs[k] = oracle.drift(tk, r0, v0, mu, fixed=False)
+        dt = spacing
+        rl, _ = oracle.hermite_log(s.m, t, rs, vs, -dt / 2, dt, len(t) - 1)  # midpoints
+        mid = t[:-1] + dt / 2
+        exact = np.array([oracle.drift(tm, r0, v0, mu, fixed=False)[1] for tm in mid])
+        errs.append(np.abs(rl - exact).max())
+    for coarse, fine in zip(errs, errs[1:]):
+        assert 12.0 < coarse / fine < 20.0, errs
+
+
+def test_hermite_rejects_short_history(oracle):
+    from oracle.pyoracle import OracleError
+    with pytest.raises(OracleError):
+        oracle.hermite_log(np.array([MU, 1e-6]), np.array([0.0]), np.zeros((1, 1, 3)), np.zeros((1, 1, 3)),
+                           0.0, 1.0, 1)
