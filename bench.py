@@ -45,14 +45,14 @@ HEADLINE_FORM = "fast"
 # FP64 FLOPs per particle-step counted by rocprofv3 on this build
 # (64 x (ADD_F64 + MUL_F64 + TRANS_F64 + 2 FMA_F64) per wave instruction,
 # summed over the block kernel and divided by its particle-steps); see
-# profiles/wh_block_counters_200k.md.  Used with the live rate to report
+# profiles/wh_block_200k_counters.md.  Used with the live rate to report
 # the counter-based share of FP64 vector peak.
-COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 362.0, "mixed": 320.0}
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 322.0, "mixed": 298.0}
 
 # HBM bytes per time-block of the fast form at this workload, from
 # rocprofv3 FETCH_SIZE (x 2.0, the factor a copy kernel of known size needs)
 # and WRITE_SIZE (x 1.0); see profiles/wh_block_200k_counters.md.
-TRAFFIC_BYTES_PER_BLOCK = {"fast": 38.3e6}
+TRAFFIC_BYTES_PER_BLOCK = {"fast": 38.7e6}
 
 
 def copy_probe(n: int = 1_000_000, repeats: int = 3) -> None:
@@ -168,9 +168,40 @@ def element_differences(mu: float, ref: dict, other: dict) -> dict:
     de = np.abs(e1[:, 1] - e0[:, 1])
     di = np.abs(e1[:, 2] - e0[:, 2])
     den = np.abs(energy(r1[keep], v1[keep]) - energy(r0[keep], v0[keep])) / np.abs(energy(r0[keep], v0[keep]))
+    dr = np.linalg.norm(r1[keep] - r0[keep], axis=1) / np.linalg.norm(r0[keep], axis=1)
+    dv = np.linalg.norm(v1[keep] - v0[keep], axis=1) / np.linalg.norm(v0[keep], axis=1)
     stat = lambda x: {"max": float(x.max()), "median": float(np.median(x))}  # noqa: E731
-    return {"n": int(keep.sum()), "da_over_a": stat(da), "de": stat(de), "di_rad": stat(di),
+    return {"n": int(keep.sum()), "dr_rel": stat(dr), "dv_rel": stat(dv), "da_over_a": stat(da), "de": stat(de),
+            "di_rad": stat(di),
             "denergy_rel": stat(den), "flags_differ": int(np.sum(f0 != f1))}
+
+
+def oracle_sample_run(inputs, n_sample: int, n_blocks: int) -> dict:
+    """The CPU oracle (restatement of the reference's src/wh.cpp) over the
+    first n_sample particles for n_blocks blocks on the same planet logs;
+    flagged particles stop, as on the GPU."""
+    from oracle.pyoracle import Oracle
+
+    s, hist, rh = inputs
+    o = Oracle()
+    n = n_sample
+    pr0, h00 = o.hermite_log(hist.m, hist.t, hist.r, hist.v, -DT, DT, 1)
+    r, v = s.pr[:n].copy(), s.pv[:n].copy()
+    a, flags = o.particle_accels(s.m, rh, OUTER, pr0[0], h00[0], r, np.zeros(n), s.ids[:n])
+    dti = None
+    for k in range(n_blocks):
+        rl, h0 = o.hermite_log(hist.m, hist.t, hist.r, hist.v, k * TBSIZE * DT, DT, TBSIZE)
+        out = o.particle_block(s.m, rh, OUTER, rl, h0, k * TBSIZE * DT, DT, r, v, a, flags, s.ids[:n], dti)
+        if out["rc"] != 0:
+            raise RuntimeError(f"oracle failed: {o.err()}")
+        r, v, a, flags, dti = out["r"], out["v"], out["a"], out["flags"], out["dti"]
+    return {"id": s.ids[:n].copy(), "r": r, "v": v, "flags": flags}
+
+
+def subset_by_id(d: dict, ids: np.ndarray) -> dict:
+    order = np.argsort(d["id"])
+    pos = order[np.searchsorted(d["id"][order], ids)]
+    return {k: d[k][pos] for k in ("id", "r", "v", "flags")}
 
 
 def cpu_baseline(inputs, sample_particles: int):
@@ -210,6 +241,8 @@ def main(argv=None):
     ap.add_argument("--waves-per-simd", type=int, default=3, help="resident-wave cap (0: none)")
     ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--oracle-sample", type=int, default=256,
+                    help="particles also run through the CPU oracle for the precision report (0: none)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -301,6 +334,12 @@ def main(argv=None):
         line["precision_vs_fp64"] = {
             "after_steps": (args.warmup + args.steps) * TBSIZE,
             **{f: element_differences(inputs[0].m[0], finals["fp64"], finals[f]) for f in finals if f != "fp64"}}
+    if args.oracle_sample > 0:
+        n_s = min(args.oracle_sample, args.particles)
+        ref = oracle_sample_run(inputs, n_s, args.warmup + args.steps)
+        line["precision_vs_oracle"] = {
+            "after_steps": (args.warmup + args.steps) * TBSIZE, "sample": n_s,
+            **{f: element_differences(inputs[0].m[0], ref, subset_by_id(finals[f], ref["id"])) for f in finals}}
     if rank == 0 and not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs, args.cpu_sample)
     if rank == 0:

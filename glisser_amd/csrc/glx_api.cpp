@@ -214,7 +214,6 @@ int fill_consts(uint32_t n_bodies, const double* m, const uint32_t* id, const do
         pc->m[j] = m[j];
         pc->rh2[j] = rh[j] * rh[j];
         pc->id[j] = id ? id[j] : j;
-        pc->mf[j] = (float)m[j];
     }
     return GLX_OK;
 }
@@ -427,9 +426,6 @@ BlockArgs block_args(glx_ctx* c, const PlanetBuf& b, double dt, double outer_rad
     A.inv_sqrt_mu = 1.0 / A.sqrt_mu;
     A.outer2 = outer_radius * outer_radius;
     A.pc = b.pc;
-    A.dominant = 1;
-    for (uint32_t j = 2; j < b.n_bodies; j++)
-        if (b.pc.m[j] > b.pc.m[A.dominant]) A.dominant = j;
     return A;
 }
 

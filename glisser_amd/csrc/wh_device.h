@@ -34,7 +34,6 @@ struct PlanetConsts {
     double m[GLX_MAX_BODIES];    // GM of each body, m[0] = Sun
     double rh2[GLX_MAX_BODIES];  // squared encounter radius (rh[0]: Sun cull radius)
     uint32_t id[GLX_MAX_BODIES]; // planet id written into the flag high byte
-    float mf[GLX_MAX_BODIES];    // GM in FP32 (dominant-term selection only)
 };
 
 // Arguments of one time-block launch.  Particle arrays are SoA; `log` is
@@ -59,7 +58,6 @@ struct BlockArgs {
     uint32_t n_steps;
     uint32_t chunk;     // steps staged in LDS at a time
     uint32_t n_bodies;  // used by the generic (NB == 0) instantiation
-    uint32_t dominant;  // MIXED form: planet whose term stays FP64 (heaviest)
     double dt;
     double half_dt;
     double mu;
@@ -72,6 +70,7 @@ struct BlockArgs {
 
 struct Particle {
     double rx, ry, rz, vx, vy, vz, ax, ay, az;
+    double r2;  // |r|^2 carried between steps (FAST / MIXED forms only)
     uint32_t flags;
 };
 

@@ -15,17 +15,17 @@
 //     instead of re-evaluating sin/cos; cos E - 1 and E - dM are carried
 //     explicitly, so f, g and the Kepler residual avoid cancellation;
 //   * the Kepler solve starts from one Newton step off E = dM (instead of
-//     the reference's series guess) and takes exactly two more corrections,
-//     accepted when the last is below 1e-14 (the convergence test of the
-//     reference's early-exit mode, src/wh.cu:60-71);
+//     the reference's series guess) and takes exactly one more correction,
+//     accepted when the error it can leave is below the 1e-14 tolerance of
+//     the reference's convergence test (src/wh.cu:60-71);
 //   * every shortcut is checked per lane; the few lanes outside them
 //     (large steps, slow convergence, unbound orbits) redo the step with the
 //     reference-arithmetic drift, so results and flags stay defined for any
 //     input;
 //   * FMA contraction is allowed.
-// MIXED additionally evaluates the r^-3 of every planet but the heaviest
-// with the FP32 reciprocal square root (|dr|^2 and the encounter test stay
-// FP64) and accumulates the acceleration in FP64.
+// MIXED additionally starts each planet's 1/|dr| from the FP32 reciprocal
+// square root and refines it with one FP64 Newton step; everything else,
+// the accumulation included, stays FP64.
 #pragma once
 
 #include "wh_device.h"
@@ -93,26 +93,27 @@ __device__ __forceinline__ void rotate_taylor(double d, double& s, double& cm1) 
     cm1 = c_new;
 }
 
-// Kepler drift, FAST form.  inv_mu, sqrt_mu, inv_sqrt_mu: host constants.
+// Kepler drift, FAST form.  inv_mu, sqrt_mu, inv_sqrt_mu: host constants;
+// p.r2 must hold |r|^2 (fma(z, z, fma(y, y, x * x))) of the current position.
 //
 // Straight-line path for the common case, then one wave vote: lanes for
-// which a shortcut does not hold (|dM| > 0.1, a start above 0.02, a first
-// correction above 1e-5, a second correction not below 1e-14, an unbound
-// orbit) redo the step with the reference-arithmetic drift from their saved
-// state, which also sets their 0x04 / 0x08 flags.
+// which a shortcut does not hold (|dM| > 0.1, a start above 0.02, a solve
+// not provably within 1e-14, an unbound orbit) redo the step with the
+// reference-arithmetic drift from their saved state, which also sets their
+// 0x04 / 0x08 flags.
 __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, double inv_mu, double sqrt_mu,
                                            double inv_sqrt_mu) {
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
     const double vx0 = p.vx, vy0 = p.vy, vz0 = p.vz;
-    const double r2 = fma(rz0, rz0, fma(ry0, ry0, rx0 * rx0));
+    const double r2 = p.r2;  // |r|^2, left by the previous step's planet sum
     const double v2 = fma(vz0, vz0, fma(vy0, vy0, vx0 * vx0));
     const double vdotr = fma(vz0, rz0, fma(vy0, ry0, vx0 * rx0));
     const double idist = rsqrt_nr(r2);
     const double dist = r2 * idist;
-    const double energy = fma(v2, 0.5, -mu * idist);
-    const double ia = -2.0 * energy * inv_mu;      // 1/a
-    const double a = rcp_nr(ia);
+    // 1/a = -2E/mu = 2/|r| - v^2/mu; bound orbits have 1/a > 0
+    const double ia = fma(-v2, inv_mu, 2.0 * idist);
     const double rs_ia = rsqrt_nr(ia);              // sqrt(a)
+    const double a = rs_ia * rs_ia;
     const double n_ = sqrt_mu * ia * (ia * rs_ia);  // sqrt(mu / a^3)
     const double in_ = inv_sqrt_mu * a * rs_ia;     // 1/n
     const double ecosEo = fma(-dist, ia, 1.0);
@@ -126,22 +127,27 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     // Kepler residual F(E) = E - ecosEo sin E + esinEo (1 - cos E) - dM and
     // F'(E) = 1 - ecosEo cos E + esinEo sin E, written with dd = E - dM and
     // (s, cm1) = (sin E, cos E - 1).  The start is one Newton step from
-    // E = dM (error ~ e^3 dM^2 / 2) instead of the reference's series guess
-    // (error ~ e^2 dM), so two more corrections reach the 1e-14 tolerance.
+    // E = dM (error ~ e^3 dM^2 / 2, against ~ e^2 dM for the reference's
+    // series guess); the raw reciprocal suffices there because the next
+    // Newton step corrects any error of the start quadratically.
     double f = fma(-esinEo, cm1, -ecosEo * s);
     double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
-    double dd = -f * rcp_nr(fp);
-    bool bad = !(energy < 0.0) || !(fabs(dM) <= 0.1) || !(fabs(dd) <= 0.02);
+    double dd = -f * __builtin_amdgcn_rcp(fp);
+    bool bad = !(ia > 0.0) || !(fabs(dM) <= 0.1) || !(fabs(dd) <= 0.02);
     rotate_taylor(dd, s, cm1);
-    // first correction: the reciprocal with one Newton step (relative error
-    // ~2e-15) is exact enough for a correction of at most 1e-5
+    // One more Newton correction.  It leaves an error of at most
+    // |F''| d1^2 / (2 F') <= d1^2 / (2 F') (|F''| <= e < 1), so the solve
+    // is accepted when that bound is below the reference's 1e-14 tolerance;
+    // the reciprocal with one Newton step (relative error ~2e-15) is exact
+    // enough for a correction that small.
     f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
     fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     const double d1 = -f * rcp_nr1(fp);
     dd += d1;
-    bad |= !(fabs(d1) <= 1e-5);
+    bad |= !(d1 * d1 < (2.0 * kTolKep) * fp);
     {
-        // sin d1 = d1 - d1^3/6, cos d1 - 1 = -d1^2/2 + d1^4/24 (exact to rounding for |d1| <= 1e-5)
+        // sin d1 = d1 - d1^3/6, cos d1 - 1 = -d1^2/2 + d1^4/24 (exact to
+        // rounding for the |d1| < 2e-7 accepted here)
         const double d2_ = d1 * d1;
         const double sd = fma(d1 * d2_, -1.0 / 6.0, d1);
         const double cd = d2_ * fma(d2_, 1.0 / 24.0, -0.5);
@@ -150,16 +156,6 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
         s = s_new;
         cm1 = c_new;
     }
-    // second correction: tiny, the raw reciprocal estimate suffices
-    f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
-    fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
-    const double d2 = -f * __builtin_amdgcn_rcp(fp);
-    dd += d2;
-    bad |= !(fabs(d2) < kTolKep);
-    // |d2| < 1e-14: the first-order rotation is exact to rounding
-    const double s2 = fma(cm1, d2, s + d2);
-    cm1 = fma(-s, d2, cm1);
-    s = s2;
     const double dE = dM + dd;
     fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     const double ifp = rcp_nr(fp);
@@ -187,21 +183,43 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     }
 }
 
-// Planet sum, FAST form: r^-3 from one refined FP64 reciprocal square root.
-template <int NB>
-__device__ __forceinline__ void accel_fast(Particle& p, const double* __restrict__ row, const BlockArgs& A,
+// Planet sum and flag tests, FAST and MIXED forms.
+//
+// r^-3 of each planet: FAST takes 1/|dr| from v_rsq_f64 with two FP64
+// Newton steps; MIXED starts from the FP32 reciprocal square root (v_rsq_f32,
+// ~1e-7) and takes one FP64 Newton step (error ~1e-14, below the rounding
+// of the FP64 sum), trading the FP64 transcendental and one correction for
+// two conversions.  The terms, their accumulation, |dr|^2 and every test
+// are FP64.
+//
+// The encounter, Sun and outer-boundary tests keep the reference's
+// precedence (src/wh.cu:151-170: the first planet inside its radius wins if
+// the particle is still unflagged, then the Sun and the outer test
+// overwrite), but the main path only ORs "some radius crossed" into one
+// wave vote; the exact assignment runs in the rare waves where a lane
+// crossed one.  |r|^2 of the new position is left in p.r2 for the next
+// drift (the kick in between does not move the particle).
+template <int NB, bool MIXED>
+__device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict__ row, const BlockArgs& A,
                                            uint32_t nb) {
     double ax = row[0], ay = row[1], az = row[2];
     const uint32_t nbodies = NB > 0 ? (uint32_t)NB : nb;
+    uint64_t close = 0;  // wave mask: lanes inside some planet's radius
 #pragma unroll
     for (uint32_t j = 1; j < (NB > 0 ? (uint32_t)NB : (uint32_t)GLX_MAX_BODIES); j++) {
-        if (NB == 0 && j >= nbodies) break;
+        if (NB == 0 && j >= nbodies) continue;  // wave-uniform
         const double dx = p.rx - row[3 * j];
         const double dy = p.ry - row[3 * j + 1];
         const double dz = p.rz - row[3 * j + 2];
         const double rad = fma(dz, dz, fma(dy, dy, dx * dx));
-        if (rad < A.pc.rh2[j] && p.flags == 0) p.flags = ((A.pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
-        const double y = rsqrt_nr(rad);
+        close |= __ballot(rad < A.pc.rh2[j]);
+        double y;
+        if constexpr (MIXED) {
+            y = (double)__builtin_amdgcn_rsqf((float)rad);
+            y = fma(y, fma(-0.5 * rad, y * y, 0.5), y);
+        } else {
+            y = rsqrt_nr(rad);
+        }
         const double fac = A.pc.m[j] * (y * y * y);
         ax = fma(-dx, fac, ax);
         ay = fma(-dy, fac, ay);
@@ -210,47 +228,23 @@ __device__ __forceinline__ void accel_fast(Particle& p, const double* __restrict
     p.ax = ax;
     p.ay = ay;
     p.az = az;
-    const double rad = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
-    if (rad < A.pc.rh2[0]) p.flags = kFlagEncounter;
-    if (rad > A.outer2) p.flags = kFlagOut;
-}
-
-// Planet sum, MIXED form: FP32 r^-3 for every planet except the dominant
-// one (the heaviest, chosen on the host), whose term is evaluated with the
-// refined FP64 reciprocal square root; all terms accumulate in FP64 and
-// |dr|^2 and the encounter test stay FP64.
-template <int NB>
-__device__ __forceinline__ void accel_mixed(Particle& p, const double* __restrict__ row, const BlockArgs& A,
-                                            uint32_t nb) {
-    double ax = row[0], ay = row[1], az = row[2];
-    const uint32_t nbodies = NB > 0 ? (uint32_t)NB : nb;
-#pragma unroll
-    for (uint32_t j = 1; j < (NB > 0 ? (uint32_t)NB : (uint32_t)GLX_MAX_BODIES); j++) {
-        if (NB == 0 && j >= nbodies) break;
-        const double dx = p.rx - row[3 * j];
-        const double dy = p.ry - row[3 * j + 1];
-        const double dz = p.rz - row[3 * j + 2];
-        const double rad = fma(dz, dz, fma(dy, dy, dx * dx));
-        if (rad < A.pc.rh2[j] && p.flags == 0) p.flags = ((A.pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
-        double y3;
-        if (j == A.dominant) {  // wave-uniform branch
-            const double y = rsqrt_nr(rad);
-            y3 = y * y * y;
-        } else {
-            const float yf = __builtin_amdgcn_rsqf((float)rad);
-            y3 = (double)(yf * yf * yf);
+    const double r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
+    p.r2 = r2;
+    if ((close | __ballot(r2 < A.pc.rh2[0] || r2 > A.outer2)) != 0) {
+        if (((close >> __lane_id()) & 1) && p.flags == 0) {
+            for (uint32_t j = 1; j < nbodies; j++) {
+                const double dx = p.rx - row[3 * j];
+                const double dy = p.ry - row[3 * j + 1];
+                const double dz = p.rz - row[3 * j + 2];
+                if (fma(dz, dz, fma(dy, dy, dx * dx)) < A.pc.rh2[j]) {
+                    p.flags = ((A.pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
+                    break;
+                }
+            }
         }
-        const double fac = A.pc.m[j] * y3;
-        ax = fma(-dx, fac, ax);
-        ay = fma(-dy, fac, ay);
-        az = fma(-dz, fac, az);
+        if (r2 < A.pc.rh2[0]) p.flags = kFlagEncounter;
+        if (r2 > A.outer2) p.flags = kFlagOut;
     }
-    p.ax = ax;
-    p.ay = ay;
-    p.az = az;
-    const double rad = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
-    if (rad < A.pc.rh2[0]) p.flags = kFlagEncounter;
-    if (rad > A.outer2) p.flags = kFlagOut;
 }
 
 __device__ __forceinline__ void kick_fma(Particle& p, double half_dt) {

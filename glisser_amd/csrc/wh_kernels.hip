@@ -41,6 +41,7 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         p.rx = p.ry = p.rz = p.vx = p.vy = p.vz = p.ax = p.ay = p.az = 0.0;
         p.flags = 0xFFFFu;  // padding lanes never step
     }
+    p.r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
 
     for (uint32_t cs = 0; cs < A.n_steps; cs += A.chunk) {
         const uint32_t cn = min(A.chunk, A.n_steps - cs);
@@ -60,10 +61,7 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
                 } else {
                     kick_fma(p, A.half_dt);
                     drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-                    if constexpr (FORM == GLX_FORM_MIXED)
-                        accel_mixed<NB>(p, rowp, A, nb);
-                    else
-                        accel_fast<NB>(p, rowp, A, nb);
+                    accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowp, A, nb);
                     kick_fma(p, A.half_dt);
                 }
                 dti = A.step_offset + cs + s + 1;
