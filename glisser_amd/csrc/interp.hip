@@ -20,6 +20,7 @@
 // few microseconds, so it stays FP64 in every kernel form.
 #include <hip/hip_runtime.h>
 
+#include "interp_device.h"
 #include "launch.h"
 
 namespace glx {
@@ -28,61 +29,10 @@ namespace {
 
 __global__ __launch_bounds__(64) void hermite_rows_kernel(HistoryView H, const double* times, double t_start,
                                                           double dt, uint32_t n_steps, double* out) {
-#pragma clang fp contract(off)
     const uint32_t s = blockIdx.x * 64 + threadIdx.x;
     if (s >= n_steps) return;
-    const double t = times ? times[s] : t_start + (double)(s + 1) * dt;
-    // Last sample with t_k <= t, within [0, n_samples - 2].
-    uint64_t lo = 0, hi = H.n_samples - 1;
-    while (hi - lo > 1) {
-        const uint64_t mid = lo + (hi - lo) / 2;
-        if (H.t[mid] <= t)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    const double h = H.t[lo + 1] - H.t[lo];
-    const double u = (t - H.t[lo]) / h;
-    const double u2 = u * u, u3 = u2 * u;
-    const double h00 = 2.0 * u3 - 3.0 * u2 + 1.0;
-    const double h10 = u3 - 2.0 * u2 + u;
-    const double h01 = 3.0 * u2 - 2.0 * u3;
-    const double h11 = u3 - u2;
-    const double w10 = h10 * h, w11 = h11 * h;
-
-    const uint32_t np = H.n_bodies - 1;
-    double* row = out + (size_t)s * H.n_bodies * 3;
-    const double* r0 = H.r + lo * np * 3;
-    const double* v0 = H.v + lo * np * 3;
-    const double* r1 = r0 + (size_t)np * 3;
-    const double* v1 = v0 + (size_t)np * 3;
-    double hx = 0, hy = 0, hz = 0, cx = 0, cy = 0, cz = 0;
-    for (uint32_t j = 0; j < np; j++) {
-        const uint32_t o = 3 * j;
-        const double x = h00 * r0[o] + w10 * v0[o] + h01 * r1[o] + w11 * v1[o];
-        const double y = h00 * r0[o + 1] + w10 * v0[o + 1] + h01 * r1[o + 1] + w11 * v1[o + 1];
-        const double z = h00 * r0[o + 2] + w10 * v0[o + 2] + h01 * r1[o + 2] + w11 * v1[o + 2];
-        row[3 + o] = x;
-        row[4 + o] = y;
-        row[5 + o] = z;
-        // load_h0: the first planet's term starts h0, the others accumulate
-        // separately and are added at the end.
-        const double r2 = x * x + y * y + z * z;
-        const double inv_r3 = 1.0 / (sqrt(r2) * r2);
-        if (j == 0) {
-            hx = -x * H.m[1] * inv_r3;
-            hy = -y * H.m[1] * inv_r3;
-            hz = -z * H.m[1] * inv_r3;
-        } else {
-            const double mfac = H.m[j + 1] * inv_r3;
-            cx -= x * mfac;
-            cy -= y * mfac;
-            cz -= z * mfac;
-        }
-    }
-    row[0] = hx + cx;
-    row[1] = hy + cy;
-    row[2] = hz + cz;
+    const double t = times ? times[s] : step_time(t_start, s, dt);
+    hermite_row(H.t, H.r, H.v, H.n_samples, H.n_bodies, H.m, t, out + (size_t)s * H.n_bodies * 3);
 }
 
 }  // namespace

@@ -138,31 +138,33 @@ __device__ __forceinline__ void kick(Particle& p, double half_dt) {
 
 // Acceleration from the step's planet row plus the encounter / cull /
 // outer-limit tests (src/wh.cu:142-170), in FP64 with IEEE divide/sqrt.
-template <int NB>
-__device__ __forceinline__ void accel_fp64(Particle& p, const double* __restrict__ row, const BlockArgs& A,
-                                           uint32_t nb) {
+// NB > 0: compile-time body count; NB == 0: nb bodies at run time, at most
+// NMAX.
+template <int NB, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES)>
+__device__ __forceinline__ void accel_fp64(Particle& p, const double* __restrict__ row, const PlanetConsts& pc,
+                                           double outer2, uint32_t nb) {
 #pragma clang fp contract(off)
     p.ax = row[0];
     p.ay = row[1];
     p.az = row[2];
     const uint32_t nbodies = NB > 0 ? (uint32_t)NB : nb;
 #pragma unroll
-    for (uint32_t j = 1; j < (NB > 0 ? (uint32_t)NB : (uint32_t)GLX_MAX_BODIES); j++) {
+    for (uint32_t j = 1; j < (uint32_t)NMAX; j++) {
         if (NB == 0 && j >= nbodies) break;
         const double dx = p.rx - row[3 * j];
         const double dy = p.ry - row[3 * j + 1];
         const double dz = p.rz - row[3 * j + 2];
         const double rad = dx * dx + dy * dy + dz * dz;
-        if (rad < A.pc.rh2[j] && p.flags == 0) p.flags = ((A.pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
+        if (rad < pc.rh2[j] && p.flags == 0) p.flags = ((pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
         const double inv3 = 1. / (rad * sqrt(rad));
-        const double fac = A.pc.m[j] * inv3;
+        const double fac = pc.m[j] * inv3;
         p.ax -= dx * fac;
         p.ay -= dy * fac;
         p.az -= dz * fac;
     }
     const double rad = p.rx * p.rx + p.ry * p.ry + p.rz * p.rz;
-    if (rad < A.pc.rh2[0]) p.flags = kFlagEncounter;
-    if (rad > A.outer2) p.flags = kFlagOut;
+    if (rad < pc.rh2[0]) p.flags = kFlagEncounter;
+    if (rad > outer2) p.flags = kFlagOut;
 }
 
 }  // namespace glx

@@ -10,14 +10,15 @@
 //     the result feeds the state, none where it only scales a Newton
 //     correction;
 //   * sin/cos of the mean-anomaly step come from a five-term Taylor
-//     polynomial (|dM| <= 0.1); the first guess and each Newton correction
+//     polynomial (|dM| <= 0.2); the first guess and each Newton correction
 //     rotate (sin E, cos E - 1) by the small angle with a short series
 //     instead of re-evaluating sin/cos; cos E - 1 and E - dM are carried
 //     explicitly, so f, g and the Kepler residual avoid cancellation;
 //   * the Kepler solve starts from one Newton step off E = dM (instead of
-//     the reference's series guess) and takes exactly one more correction,
-//     accepted when the error it can leave is below the 1e-14 tolerance of
-//     the reference's convergence test (src/wh.cu:60-71);
+//     the reference's series guess) and takes one more correction (two in
+//     waves where some lane needs it), accepted when the error it can leave
+//     is below the 1e-14 tolerance of the reference's convergence test
+//     (src/wh.cu:60-71);
 //   * every shortcut is checked per lane; the few lanes outside them
 //     (large steps, slow convergence, unbound orbits) redo the step with the
 //     reference-arithmetic drift, so results and flags stay defined for any
@@ -65,8 +66,8 @@ __constant__ double kSinTaylor[5] = {-1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0
 __constant__ double kCosTaylor[5] = {-1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0, -1.0 / 3628800.0};
 
 // sin x and cos x - 1 from the first N Taylor terms (Horner in x^2).  With
-// N = 5 the truncation is below 1e-20 for |x| <= 0.1; with N = 4 below
-// 1e-24 for |x| <= 0.02.
+// N = 5 the truncation is below 9e-18 for |x| <= 0.2; with N = 4 below
+// 3e-20 for |x| <= 0.05.
 template <int N>
 __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) {
     const double x2 = x * x;
@@ -81,7 +82,7 @@ __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) 
     cm1 = pc * x2;
 }
 
-// Advance (s, cm1) = (sin E, cos E - 1) to E + d (|d| <= 0.02; larger
+// Advance (s, cm1) = (sin E, cos E - 1) to E + d (|d| <= 0.05; larger
 // corrections are routed to the fallback by the caller).
 __device__ __forceinline__ void rotate_taylor(double d, double& s, double& cm1) {
     double sd, cd;
@@ -93,14 +94,44 @@ __device__ __forceinline__ void rotate_taylor(double d, double& s, double& cm1) 
     cm1 = c_new;
 }
 
+// Validity bounds of the FAST drift's shortcuts: the five-term Taylor
+// sin/cos of dM (truncation below 9e-18 absolute at |dM| = 0.2) and the
+// four-term rotation by the start correction (below 3e-20 at 0.05).
+constexpr double kMaxDM = 0.2;
+constexpr double kMaxStart = 0.05;
+
+// One Newton correction of the Kepler solve in the (dd, sin E, cos E - 1)
+// form; returns whether the error it leaves is provably below 1e-14
+// (d^2 < 2e-14 F').  Corrections above 1e-3 are left unapplied and
+// reported as not converged (the two-term rotation is exact to rounding
+// only below that).
+__device__ __forceinline__ bool newton_step(double esinEo, double ecosEo, double omec, double& dd, double& s,
+                                            double& cm1) {
+    const double f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
+    const double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
+    double d = -f * rcp_nr1(fp);
+    const bool small = fabs(d) <= 1e-3;
+    d = small ? d : 0.0;
+    dd += d;
+    // sin d = d - d^3/6, cos d - 1 = -d^2/2 + d^4/24 (truncation below 1e-17 for |d| <= 1e-3)
+    const double d2 = d * d;
+    const double sd = fma(d * d2, -1.0 / 6.0, d);
+    const double cd = d2 * fma(d2, 1.0 / 24.0, -0.5);
+    const double s_new = s + sd + fma(s, cd, cm1 * sd);
+    const double c_new = cm1 + cd + fma(cm1, cd, -s * sd);
+    s = s_new;
+    cm1 = c_new;
+    return small && d2 < (2.0 * kTolKep) * fp;
+}
+
 // Kepler drift, FAST form.  inv_mu, sqrt_mu, inv_sqrt_mu: host constants;
 // p.r2 must hold |r|^2 (fma(z, z, fma(y, y, x * x))) of the current position.
 //
 // Straight-line path for the common case, then one wave vote: lanes for
-// which a shortcut does not hold (|dM| > 0.1, a start above 0.02, a solve
-// not provably within 1e-14, an unbound orbit) redo the step with the
-// reference-arithmetic drift from their saved state, which also sets their
-// 0x04 / 0x08 flags.
+// which a shortcut does not hold (|dM| > 0.2, a start above 0.05, a solve
+// not provably within 1e-14 after two corrections, an unbound orbit) redo
+// the step with the reference-arithmetic drift from their saved state,
+// which also sets their 0x04 / 0x08 flags.
 __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, double inv_mu, double sqrt_mu,
                                            double inv_sqrt_mu) {
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
@@ -118,7 +149,7 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     const double in_ = inv_sqrt_mu * a * rs_ia;     // 1/n
     const double ecosEo = fma(-dist, ia, 1.0);
     const double esinEo = vdotr * in_ * ia * ia;
-    // |dM| <= 0.1 on this path, so no full turns are removed and the time
+    // |dM| <= 0.2 on this path, so no full turns are removed and the time
     // left to advance, dM / n, is dt itself
     const double dM = dt * n_;
     double s, cm1;
@@ -133,29 +164,18 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     double f = fma(-esinEo, cm1, -ecosEo * s);
     double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     double dd = -f * __builtin_amdgcn_rcp(fp);
-    bool bad = !(ia > 0.0) || !(fabs(dM) <= 0.1) || !(fabs(dd) <= 0.02);
+    bool bad = !(ia > 0.0) || !(fabs(dM) <= kMaxDM) || !(fabs(dd) <= kMaxStart);
     rotate_taylor(dd, s, cm1);
-    // One more Newton correction.  It leaves an error of at most
-    // |F''| d1^2 / (2 F') <= d1^2 / (2 F') (|F''| <= e < 1), so the solve
-    // is accepted when that bound is below the reference's 1e-14 tolerance;
-    // the reciprocal with one Newton step (relative error ~2e-15) is exact
-    // enough for a correction that small.
-    f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
-    fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
-    const double d1 = -f * rcp_nr1(fp);
-    dd += d1;
-    bad |= !(d1 * d1 < (2.0 * kTolKep) * fp);
-    {
-        // sin d1 = d1 - d1^3/6, cos d1 - 1 = -d1^2/2 + d1^4/24 (exact to
-        // rounding for the |d1| < 2e-7 accepted here)
-        const double d2_ = d1 * d1;
-        const double sd = fma(d1 * d2_, -1.0 / 6.0, d1);
-        const double cd = d2_ * fma(d2_, 1.0 / 24.0, -0.5);
-        const double s_new = s + sd + fma(s, cd, cm1 * sd);
-        const double c_new = cm1 + cd + fma(cm1, cd, -s * sd);
-        s = s_new;
-        cm1 = c_new;
-    }
+    // Newton corrections.  One leaves an error of at most |F''| d^2 / (2 F')
+    // <= d^2 / (2 F') (|F''| <= e < 1), so the solve is accepted as soon as
+    // that bound is below the reference's 1e-14 tolerance.  One correction
+    // suffices when e dM is small (the config of most swarms); a second runs
+    // only in waves where some lane needs it.  The reciprocal with one
+    // Newton step (relative error ~2e-15) is exact enough for corrections
+    // that small.
+    bool conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1);
+    if (__any(!conv)) conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1);
+    bad |= !conv;
     const double dE = dM + dd;
     fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     const double ifp = rcp_nr(fp);
@@ -199,20 +219,20 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
 // wave vote; the exact assignment runs in the rare waves where a lane
 // crossed one.  |r|^2 of the new position is left in p.r2 for the next
 // drift (the kick in between does not move the particle).
-template <int NB, bool MIXED>
-__device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict__ row, const BlockArgs& A,
-                                           uint32_t nb) {
+template <int NB, bool MIXED, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES)>
+__device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict__ row, const PlanetConsts& pc,
+                                           double outer2, uint32_t nb) {
     double ax = row[0], ay = row[1], az = row[2];
     const uint32_t nbodies = NB > 0 ? (uint32_t)NB : nb;
     uint64_t close = 0;  // wave mask: lanes inside some planet's radius
 #pragma unroll
-    for (uint32_t j = 1; j < (NB > 0 ? (uint32_t)NB : (uint32_t)GLX_MAX_BODIES); j++) {
+    for (uint32_t j = 1; j < (uint32_t)NMAX; j++) {
         if (NB == 0 && j >= nbodies) continue;  // wave-uniform
         const double dx = p.rx - row[3 * j];
         const double dy = p.ry - row[3 * j + 1];
         const double dz = p.rz - row[3 * j + 2];
         const double rad = fma(dz, dz, fma(dy, dy, dx * dx));
-        close |= __ballot(rad < A.pc.rh2[j]);
+        close |= __ballot(rad < pc.rh2[j]);
         double y;
         if constexpr (MIXED) {
             y = (double)__builtin_amdgcn_rsqf((float)rad);
@@ -220,7 +240,7 @@ __device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict
         } else {
             y = rsqrt_nr(rad);
         }
-        const double fac = A.pc.m[j] * (y * y * y);
+        const double fac = pc.m[j] * (y * y * y);
         ax = fma(-dx, fac, ax);
         ay = fma(-dy, fac, ay);
         az = fma(-dz, fac, az);
@@ -230,20 +250,20 @@ __device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict
     p.az = az;
     const double r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
     p.r2 = r2;
-    if ((close | __ballot(r2 < A.pc.rh2[0] || r2 > A.outer2)) != 0) {
+    if ((close | __ballot(r2 < pc.rh2[0] || r2 > outer2)) != 0) {
         if (((close >> __lane_id()) & 1) && p.flags == 0) {
             for (uint32_t j = 1; j < nbodies; j++) {
                 const double dx = p.rx - row[3 * j];
                 const double dy = p.ry - row[3 * j + 1];
                 const double dz = p.rz - row[3 * j + 2];
-                if (fma(dz, dz, fma(dy, dy, dx * dx)) < A.pc.rh2[j]) {
-                    p.flags = ((A.pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
+                if (fma(dz, dz, fma(dy, dy, dx * dx)) < pc.rh2[j]) {
+                    p.flags = ((pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
                     break;
                 }
             }
         }
-        if (r2 < A.pc.rh2[0]) p.flags = kFlagEncounter;
-        if (r2 > A.outer2) p.flags = kFlagOut;
+        if (r2 < pc.rh2[0]) p.flags = kFlagEncounter;
+        if (r2 > outer2) p.flags = kFlagOut;
     }
 }
 

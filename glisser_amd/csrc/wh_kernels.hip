@@ -56,12 +56,12 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
                 if constexpr (FORM == GLX_FORM_FP64) {
                     kick(p, A.half_dt);
                     drift_fp64(p, A.dt, A.mu);
-                    accel_fp64<NB>(p, rowp, A, nb);
+                    accel_fp64<NB>(p, rowp, A.pc, A.outer2, nb);
                     kick(p, A.half_dt);
                 } else {
                     kick_fma(p, A.half_dt);
                     drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-                    accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowp, A, nb);
+                    accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowp, A.pc, A.outer2, nb);
                     kick_fma(p, A.half_dt);
                 }
                 dti = A.step_offset + cs + s + 1;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void init_accel_kernel(BlockArgs A) {
     p.ry = A.ry[i];
     p.rz = A.rz[i];
     p.flags = A.flags[i];
-    accel_fp64<NB>(p, lds, A, nb);
+    accel_fp64<NB>(p, lds, A.pc, A.outer2, nb);
     A.ax[i] = p.ax;
     A.ay[i] = p.ay;
     A.az[i] = p.az;

@@ -43,6 +43,39 @@ class Params(C.Structure):
     ]
 
 
+class SystemDesc(C.Structure):
+    """glx_system: one planetary system of a batch."""
+    _fields_ = [
+        ("n_bodies", C.c_uint32),
+        ("n_particles", C.c_uint32),
+        ("t0", C.c_double),
+        ("dt", C.c_double),
+        ("outer_radius", C.c_double),
+        ("m", _dp),
+        ("id", _u32p),
+        ("rh", _dp),
+        ("n_samples", C.c_uint64),
+        ("t", _dp),
+        ("r", _dp),
+        ("v", _dp),
+    ]
+
+
+class BatchConfig(C.Structure):
+    """glx_batch_config: launch shape of a batch."""
+    _fields_ = [
+        ("steps_per_launch", C.c_uint32),
+        ("lanes", C.c_uint32),
+        ("persistent", C.c_uint32),
+        ("waves_per_simd", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+BATCH_MAX_BODIES = 9
+
 # name -> (restype, argtypes); every symbol declared in include/glisser_amd.h
 SIGNATURES = {
     "glx_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
@@ -68,6 +101,17 @@ SIGNATURES = {
     "glx_sync": (C.c_int, [C.c_void_p]),
     "glx_last_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "glx_sum_steps": (C.c_int, [C.c_void_p, C.c_uint64, _u64p]),
+    "glx_batch_create": (C.c_int, [C.c_int, C.POINTER(SystemDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "glx_batch_destroy": (C.c_int, [C.c_void_p]),
+    "glx_batch_upload": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _dp, _u16p, _u32p]),
+    "glx_batch_download": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _dp, _u16p, _u32p, _u32p]),
+    "glx_batch_init_accel": (C.c_int, [C.c_void_p]),
+    "glx_batch_set_config": (C.c_int, [C.c_void_p, C.POINTER(BatchConfig)]),
+    "glx_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(BatchConfig)]),
+    "glx_batch_run": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_float)]),
+    "glx_batch_tune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BatchConfig), C.c_uint32, C.POINTER(BatchConfig),
+                                 C.POINTER(C.c_double)]),
+    "glx_batch_sum_steps": (C.c_int, [C.c_void_p, _u64p]),
     "glx_planets_create": (C.c_int, [C.c_uint32, _dp, _dp, _dp, _u32p, C.c_double, C.c_double, C.c_double,
                                      C.POINTER(C.c_void_p)]),
     "glx_planets_destroy": (C.c_int, [C.c_void_p]),
@@ -333,6 +377,123 @@ class Planets:
         lv = np.empty(3)
         _check_host(self.lib.glx_planets_metrics(self.h, _ptr(e, _dp), _ptr(lv, _dp)))
         return float(e[0]), lv
+
+
+class Batch:
+    """Many independent systems advanced together (glx_batch_*).
+
+    systems: sequence of dicts with keys m, rh, n_particles, dt, history
+    (an object with t, r, v), optional t0 (default: first history time),
+    outer_radius (default 300) and ids (planet ids)."""
+
+    def __init__(self, systems, steps_max: int = 2048, device: int = 0):
+        self.lib = load()
+        self._keep = []
+        descs = (SystemDesc * len(systems))()
+        self.sizes = []
+        for k, y in enumerate(systems):
+            m = np.ascontiguousarray(y["m"], dtype=np.float64)
+            nb = len(m)
+            rh = np.ascontiguousarray(y["rh"], dtype=np.float64)
+            h = y["history"]
+            t = np.ascontiguousarray(h.t, dtype=np.float64)
+            r = np.ascontiguousarray(h.r, dtype=np.float64).reshape(len(t), nb - 1, 3)
+            v = np.ascontiguousarray(h.v, dtype=np.float64).reshape(len(t), nb - 1, 3)
+            ids = y.get("ids")
+            ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+            self._keep += [m, rh, t, r, v, ids]
+            d = descs[k]
+            d.n_bodies = nb
+            d.n_particles = int(y["n_particles"])
+            d.t0 = float(y.get("t0", t[0]))
+            d.dt = float(y["dt"])
+            d.outer_radius = float(y.get("outer_radius", 300.0))
+            d.m, d.rh, d.t, d.r, d.v = (_ptr(m, _dp), _ptr(rh, _dp), _ptr(t, _dp), _ptr(r, _dp), _ptr(v, _dp))
+            d.id = _ptr(ids, _u32p)
+            d.n_samples = len(t)
+            self.sizes.append(int(y["n_particles"]))
+        h = C.c_void_p()
+        rc = self.lib.glx_batch_create(device, descs, len(systems), steps_max, C.byref(h))
+        self._keep = None
+        if rc != 0:
+            raise GlxError(rc, self.lib.glx_last_error().decode())
+        self.h = h
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise GlxError(rc, self.lib.glx_last_error().decode())
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.glx_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, sys: int, r, v, flags=None, ids=None) -> None:
+        n = self.sizes[sys]
+        r = _vec3(r, n)
+        v = _vec3(v, n)
+        flags = np.zeros(n, np.uint16) if flags is None else np.ascontiguousarray(flags, dtype=np.uint16)
+        ids = np.arange(n, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+        self._check(self.lib.glx_batch_upload(self.h, sys, _ptr(r, _dp), _ptr(v, _dp), _ptr(flags, _u16p),
+                                              _ptr(ids, _u32p)))
+
+    def download(self, sys: int) -> dict:
+        n = self.sizes[sys]
+        out = {"r": np.empty((n, 3)), "v": np.empty((n, 3)), "flags": np.empty(n, np.uint16),
+               "id": np.empty(n, np.uint32), "dti": np.empty(n, np.uint32)}
+        self._check(self.lib.glx_batch_download(self.h, sys, _ptr(out["r"], _dp), _ptr(out["v"], _dp),
+                                                _ptr(out["flags"], _u16p), _ptr(out["id"], _u32p),
+                                                _ptr(out["dti"], _u32p)))
+        return out
+
+    def init_accel(self) -> None:
+        self._check(self.lib.glx_batch_init_accel(self.h))
+
+    def set_config(self, steps_per_launch: int, lanes: int = 256, persistent: int = 0,
+                   waves_per_simd: int = 0) -> None:
+        c = BatchConfig(steps_per_launch, lanes, persistent, waves_per_simd)
+        self._check(self.lib.glx_batch_set_config(self.h, C.byref(c)))
+
+    def config(self) -> dict:
+        c = BatchConfig()
+        self._check(self.lib.glx_batch_get_config(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def run(self, n_launches: int, form: str | int = "fp64") -> float:
+        f = FORMS[form] if isinstance(form, str) else int(form)
+        ms = C.c_float(0.0)
+        self._check(self.lib.glx_batch_run(self.h, n_launches, f, C.byref(ms)))
+        return float(ms.value)
+
+    def tune(self, form: str | int = "fast", candidates=None):
+        """Returns (best config dict, particle-steps/s of the best)."""
+        f = FORMS[form] if isinstance(form, str) else int(form)
+        cand = None
+        n = 0
+        if candidates:
+            cand = (BatchConfig * len(candidates))(*[BatchConfig(*c) for c in candidates])
+            n = len(candidates)
+        best = BatchConfig()
+        rate = C.c_double(0.0)
+        self._check(self.lib.glx_batch_tune(self.h, f, cand, n, C.byref(best), C.byref(rate)))
+        return best.as_dict(), float(rate.value)
+
+    def sum_steps(self) -> int:
+        out = C.c_uint64(0)
+        self._check(self.lib.glx_batch_sum_steps(self.h, C.byref(out)))
+        return int(out.value)
 
 
 def load_h0(m, r_log) -> np.ndarray:

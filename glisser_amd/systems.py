@@ -134,3 +134,67 @@ def planet_history(system: System, t_end: float, spacing: float = 365.0, substep
         pl.close()
     t = np.arange(n_samples, dtype=np.float64) * spacing
     return History(system.m.copy(), t, r, v)
+
+
+def _hill_spaced_axes(rng, mus, a_lo, a_hi, min_hill, tries=2000):
+    """Semi-major axes log-uniform in [a_lo, a_hi], sorted, with every
+    neighbouring pair at least `min_hill` mutual Hill radii apart
+    (R_H = ((mu_k + mu_k+1) / 3)^(1/3) (a_k + a_k+1) / 2); None if rejection
+    fails `tries` times."""
+    n = len(mus)
+    for _ in range(tries):
+        a = np.sort(np.exp(rng.uniform(np.log(a_lo), np.log(a_hi), n)))
+        mu = mus  # masses keep their draw order; pair them with sorted axes
+        rh = ((mu[:-1] + mu[1:]) / 3.0) ** (1.0 / 3.0) * (a[:-1] + a[1:]) / 2.0
+        if np.all(np.diff(a) >= min_hill * rh):
+            return a
+    return None
+
+
+def random_system(rng, n_particles: int, n_planets: int, mu_range=(1e-5, 1e-3), a_range=(1.0, 40.0),
+                  min_hill=8.0, e_hi=0.05, i_hi_deg=2.0):
+    """One system of the batched workload: Sun + n_planets planets with
+    log-uniform mass ratios and semi-major axes at least `min_hill` mutual
+    Hill radii apart, e < e_hi; a particle belt outside the outermost planet
+    (a in [1.3, 2.0] a_out, e < 0.1, i < 5 deg).  dt = P_inner / 20.  Returns
+    (System, dt, rh, outer_radius)."""
+    while True:
+        mus = np.exp(rng.uniform(np.log(mu_range[0]), np.log(mu_range[1]), n_planets))
+        axes = None
+        for _ in range(50):
+            axes = _hill_spaced_axes(rng, mus, a_range[0], a_range[1], min_hill)
+            if axes is not None:
+                break
+            mus = np.exp(rng.uniform(np.log(mu_range[0]), np.log(mu_range[1]), n_planets))
+        if axes is not None:
+            break
+        n_planets -= 1  # too many planets for the range at this spacing
+    m, r, v = planets(rng, mus=mus, axes=axes, e_hi=e_hi, i_hi_deg=i_hi_deg)
+    a_out = axes[-1]
+    el = _elements(rng, n_particles, 1.3 * a_out, 2.0 * a_out, 0.1, 5.0)
+    pr, pv = native.from_elements(np.full(n_particles, m[0]), el, mean_anomaly=True)
+    p_inner = 2 * np.pi * np.sqrt(axes[0] ** 3 / (m[0] + m[1]))
+    dt = p_inner / 20.0
+    rh = np.concatenate([[0.25 * axes[0]], 3.5 * axes * (mus / 3.0) ** (1.0 / 3.0)])
+    s = System(m, r, v, pr, pv, np.arange(n_particles, dtype=np.uint32), 0,
+               {"kind": "batch_member", "axes": axes, "mus": mus})
+    return s, dt, rh, 10.0 * a_out
+
+
+def batch_systems(n_systems: int = 64, seed: int = 3, n_range=(1e3, 1e6), planets_range=(2, 8),
+                  n_steps: int = 4096, substeps: int = 2):
+    """The batched workload: n_systems independent systems with 2-8 planets
+    each (random_system), particle counts log-uniform in n_range, and a
+    planet history per system sampled every dt (host Jacobi WH at
+    dt / substeps) covering n_steps steps.  Returns a list of dicts with the
+    keys Batch() takes plus "system"."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(n_systems):
+        n = int(round(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1])))))
+        npl = int(rng.integers(planets_range[0], planets_range[1] + 1))
+        s, dt, rh, outer = random_system(rng, n, npl)
+        hist = planet_history(s, (n_steps + 1) * dt, spacing=dt, substeps=substeps, cull_radius=rh[0])
+        out.append({"system": s, "m": s.m, "rh": rh, "n_particles": n, "dt": dt, "t0": 0.0,
+                    "outer_radius": outer, "history": hist})
+    return out

@@ -187,6 +187,96 @@ int glx_last_times(glx_ctx* ctx, float* kernel_ms, float* partition_ms);
 int glx_sum_steps(glx_ctx* ctx, uint64_t n, uint64_t* total);
 
 /* ---------------------------------------------------------------------
+ * Batched runs: many independent planetary systems (different bodies,
+ * particle counts and timesteps) advanced together, one launch per
+ * time-block for all of them.  No reference counterpart: the reference
+ * integrates one system per process (src/executor.cu:598-764); a batch is
+ * what a caller would otherwise run as many single-system executors.
+ * ------------------------------------------------------------------- */
+
+typedef struct glx_batch glx_batch;
+
+#define GLX_BATCH_MAX_BODIES 9
+
+/* One system of a batch.  m, rh: n_bodies values (GM units, m[0] the Sun;
+ * rh[0] the Sun's cull radius); id: n_bodies planet ids or NULL.  The
+ * planets come from a sampled history (n_samples >= 2 strictly increasing
+ * times t, heliocentric r and v of bodies 1..n_bodies-1, n_samples x
+ * (n_bodies-1) x 3 each) interpolated on the device with cubic Hermite
+ * polynomials; step s of the batch (counted from 0) is at t0 + (s+1) dt. */
+typedef struct glx_system {
+    uint32_t n_bodies;     /* 2 .. GLX_BATCH_MAX_BODIES */
+    uint32_t n_particles;
+    double t0;
+    double dt;
+    double outer_radius;
+    const double* m;
+    const uint32_t* id;
+    const double* rh;
+    uint64_t n_samples;
+    const double* t;
+    const double* r;
+    const double* v;
+} glx_system;
+
+/* Launch shape of a batch: steps per launch (<= the batch's maximum),
+ * lanes per workgroup (64, 128 or 256), persistent (0: one workgroup per
+ * tile of `lanes` particles; 1: a fixed grid of workgroups takes tiles from
+ * a device queue), waves_per_simd (0: no cap; 1..8: resident waves per SIMD,
+ * which also sizes the persistent grid). */
+typedef struct glx_batch_config {
+    uint32_t steps_per_launch;
+    uint32_t lanes;
+    uint32_t persistent;
+    uint32_t waves_per_simd;
+} glx_batch_config;
+
+/* Create a batch on device `device` holding the systems (histories and
+ * constants are copied; the caller's arrays may be freed after the call).
+ * steps_max bounds steps_per_launch.  Particles start zeroed: upload them
+ * with glx_batch_upload, then glx_batch_init_accel. */
+int glx_batch_create(int device, const glx_system* systems, uint32_t n_systems, uint32_t steps_max,
+                     glx_batch** out);
+int glx_batch_destroy(glx_batch* b);
+
+/* Particles of system `sys` (its n_particles entries; r, v: n x 3), synchronous.
+ * On download any output may be NULL; deathtime_index counts the steps since
+ * the batch started (flagged particles keep the step count of their last step). */
+int glx_batch_upload(glx_batch* b, uint32_t sys, const double* r, const double* v, const uint16_t* flags,
+                     const uint32_t* id);
+int glx_batch_download(glx_batch* b, uint32_t sys, double* r, double* v, uint16_t* flags, uint32_t* id,
+                       uint32_t* deathtime_index);
+
+/* Initial accelerations and flags of every system at its t0 (planet rows
+ * from the history), synchronous: the batched counterpart of the
+ * helio_acc_particles<true> call in the WHIntegrator constructor
+ * (src/wh.cpp:531-543). */
+int glx_batch_init_accel(glx_batch* b);
+
+/* Set / read the launch shape.  Setting it captures the launch
+ * (interpolation, particle kernel, clock) into a HIP graph on the next run. */
+int glx_batch_set_config(glx_batch* b, const glx_batch_config* cfg);
+int glx_batch_get_config(glx_batch* b, glx_batch_config* cfg);
+
+/* Replay the captured launch n_launches times (steps_per_launch steps of
+ * every system each), then wait; *elapsed_ms (may be NULL) receives the
+ * device time between events around the replays.  GLX_EINVAL if a system's
+ * history does not cover the steps. */
+int glx_batch_run(glx_batch* b, uint32_t n_launches, int form, float* elapsed_ms);
+
+/* Autotuner: time one launch of each candidate shape (n_candidates entries;
+ * NULL / 0 = the built-in set over steps {512, 1024, 2048}, lanes {64, 128,
+ * 256}, dispatch and persistent modes) from the current state, restore the
+ * state, keep the fastest shape and write it to *best (particle-steps per
+ * second to *best_rate; either may be NULL). */
+int glx_batch_tune(glx_batch* b, int form, const glx_batch_config* candidates, uint32_t n_candidates,
+                   glx_batch_config* best, double* best_rate);
+
+/* Sum of deathtime_index over every particle of the batch (synchronous):
+ * the particle-steps done since the batch started. */
+int glx_batch_sum_steps(glx_batch* b, uint64_t* total);
+
+/* ---------------------------------------------------------------------
  * Host-side producer of the kernel's planet input and orbital conversions.
  * Built without FP contraction so it rounds like the reference CPU build.
  * ------------------------------------------------------------------- */

@@ -101,3 +101,44 @@ def test_generator_is_seeded_and_in_range():
     assert el[:, 1].max() < 0.1 + 1e-9 and el[:, 2].max() < 5 * np.pi / 180 + 1e-9
     pel = native.to_elements(a.m[0] + a.m[1:], a.r[1:], a.v[1:])
     np.testing.assert_allclose(pel[:, 0], [5.20, 9.55, 19.2, 30.1], rtol=1e-12)
+
+
+def test_batch_generator_properties():
+    """Batched workload: 2-8 planets at >= 8 mutual Hill radii, dt = P_inner / 20,
+    particles outside the outermost planet, histories sampled every dt."""
+    from glisser_amd import native, systems
+    members = systems.batch_systems(6, seed=5, n_range=(100, 300), n_steps=16)
+    for y in members:
+        s = y["system"]
+        mus = s.m[1:] / s.m[0]
+        pel = native.to_elements(s.m[0] + s.m[1:], s.r[1:], s.v[1:])
+        a = pel[:, 0]
+        assert 2 <= len(mus) <= 8 and np.all(np.diff(a) > 0)
+        rh_mut = ((mus[:-1] + mus[1:]) / 3) ** (1 / 3) * (a[:-1] + a[1:]) / 2
+        assert np.all(np.diff(a) >= 8 * rh_mut * 0.99)  # osculating a differs slightly from the drawn one
+        p_in = 2 * np.pi * np.sqrt(s.meta["axes"][0] ** 3 / (s.m[0] + s.m[1]))
+        assert abs(y["dt"] - p_in / 20) < 1e-9 * y["dt"]
+        h = y["history"]
+        np.testing.assert_allclose(np.diff(h.t), y["dt"], rtol=1e-12)
+        assert h.t[-1] >= 17 * y["dt"]
+        pa = native.to_elements(np.full(s.n, s.m[0]), s.pr, s.pv)[:, 0]
+        assert pa.min() > 1.29 * s.meta["axes"][-1]
+
+
+def test_batch_rejects_bad_systems():
+    """Descriptor validation happens on the host, before any device work."""
+    from glisser_amd import native, systems
+    s = systems.three_body(8, seed=1)
+    h = systems.planet_history(s, 400.0, spacing=100.0)
+    good = {"m": s.m, "rh": np.array([0.5, 1.0]), "n_particles": 8, "dt": 50.0, "history": h}
+    bad_cases = [
+        dict(good, dt=0.0),
+        dict(good, m=np.array([0.0, 1e-7])),
+        dict(good, history=systems.History(h.m, h.t[::-1].copy(), h.r, h.v)),
+        dict(good, t0=-1000.0),
+    ]
+    for case in bad_cases:
+        with pytest.raises(native.GlxError):
+            native.Batch([case])
+    with pytest.raises(native.GlxError):
+        native.Batch([])
