@@ -204,6 +204,43 @@ def subset_by_id(d: dict, ids: np.ndarray) -> dict:
     return {k: d[k][pos] for k in ("id", "r", "v", "flags")}
 
 
+BATCH_SYSTEMS = 64
+BATCH_LAUNCHES = 4
+
+
+def run_batch(forms, n_systems: int, launches: int, seed: int) -> dict:
+    """Batched workload: n_systems independent systems (2-8 planets each,
+    particle counts log-uniform 1e3-1e6, dt = P_inner / 20), all in one launch
+    per time-block, captured in a HIP graph.  Per form: the autotuner picks
+    steps per launch, lanes per workgroup and dispatch / persistent mode
+    (outside the timed region, state restored), then one warm-up launch and
+    `launches` timed launches."""
+    from glisser_amd import native, systems
+
+    members = systems.batch_systems(n_systems, seed=seed, n_steps=(launches + 1) * 2048)
+    n_total = sum(y["n_particles"] for y in members)
+    out = {"systems": n_systems, "particles": n_total,
+           "bodies_histogram": np.bincount([len(y["m"]) for y in members], minlength=10)[2:].tolist(),
+           "launches": launches, "forms": {}}
+    for form in forms:
+        with native.Batch(members, steps_max=2048) as b:
+            for k, y in enumerate(members):
+                s = y["system"]
+                b.upload(k, s.pr, s.pv)
+            b.init_accel()
+            cfg, tune_rate = b.tune(form)
+            b.run(1, form)
+            s0 = b.sum_steps()
+            ms = b.run(launches, form)
+            s1 = b.sum_steps()
+        out["forms"][form] = {"value": (s1 - s0) / (ms * 1e-3), "ms_per_launch": ms / launches,
+                              "steps_per_launch": cfg["steps_per_launch"], "config": cfg,
+                              "tune_rate": tune_rate, "particle_steps": s1 - s0}
+    if "fp64" in out["forms"] and "mixed" in out["forms"]:
+        out["mixed_vs_fp64"] = out["forms"]["mixed"]["value"] / out["forms"]["fp64"]["value"]
+    return out
+
+
 def cpu_baseline(inputs, sample_particles: int):
     """Reference CPU path (oracle/_ref: the reference's own src/wh.cpp built
     from its sources) or, where that library is absent, the oracle
@@ -243,6 +280,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--oracle-sample", type=int, default=256,
                     help="particles also run through the CPU oracle for the precision report (0: none)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-system workload")
+    ap.add_argument("--batch-systems", type=int, default=BATCH_SYSTEMS)
+    ap.add_argument("--batch-launches", type=int, default=BATCH_LAUNCHES)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -340,6 +380,8 @@ def main(argv=None):
         line["precision_vs_oracle"] = {
             "after_steps": (args.warmup + args.steps) * TBSIZE, "sample": n_s,
             **{f: element_differences(inputs[0].m[0], ref, subset_by_id(finals[f], ref["id"])) for f in finals}}
+    if not args.no_batch:
+        line["batch"] = run_batch(forms, args.batch_systems, args.batch_launches, args.seed + 2)
     if rank == 0 and not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs, args.cpu_sample)
     if rank == 0:
