@@ -655,9 +655,13 @@ int glx_download_planet_block(glx_ctx* c, int buf, uint32_t n_bodies, uint32_t n
     return GLX_OK;
 }
 
-int glx_init_accel(glx_ctx* c, uint32_t n_bodies, const double* m, const uint32_t* id, const double* rh,
-                   const double* planet_r, const double* h0, double outer_radius) {
-    if (!c) return fail(GLX_EINVAL, "null context");
+}  // extern "C"
+
+namespace {
+
+// Accelerations and flags of particles [begin, end) from one planet row.
+int accel_range(glx_ctx* c, uint64_t begin, uint64_t end, uint32_t n_bodies, const double* m, const uint32_t* id,
+                const double* rh, const double* planet_r, const double* h0, double outer_radius) {
     if (n_bodies < 1 || n_bodies > c->nb_max)
         return fail(GLX_EINVAL, "n_bodies %u outside [1, %u]", n_bodies, c->nb_max);
     if (!m || !rh || !h0 || (n_bodies > 1 && !planet_r)) return fail(GLX_EINVAL, "null planet array");
@@ -675,7 +679,8 @@ int glx_init_accel(glx_ctx* c, uint32_t n_bodies, const double* m, const uint32_
                            c->main_s));
     BlockArgs A = base_args(c);
     A.log = row_dev;
-    A.n = (uint32_t)c->n_alive;
+    A.first = (uint32_t)begin;
+    A.n = (uint32_t)end;
     A.n_steps = 1;
     A.n_bodies = n_bodies;
     A.outer2 = outer_radius * outer_radius;
@@ -683,6 +688,23 @@ int glx_init_accel(glx_ctx* c, uint32_t n_bodies, const double* m, const uint32_
     GLX_HIP(glx::launch_init_accel(A, c->main_s));
     GLX_HIP(hipStreamSynchronize(c->main_s));
     return GLX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glx_init_accel(glx_ctx* c, uint32_t n_bodies, const double* m, const uint32_t* id, const double* rh,
+                   const double* planet_r, const double* h0, double outer_radius) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    return accel_range(c, 0, c->n_alive, n_bodies, m, id, rh, planet_r, h0, outer_radius);
+}
+
+int glx_accel_range(glx_ctx* c, uint64_t begin, uint64_t n, uint32_t n_bodies, const double* m, const uint32_t* id,
+                    const double* rh, const double* planet_r, const double* h0, double outer_radius) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_range(c, begin, n)) return rc;
+    return accel_range(c, begin, begin + n, n_bodies, m, id, rh, planet_r, h0, outer_radius);
 }
 
 int glx_kill_encounters(glx_ctx* c) {

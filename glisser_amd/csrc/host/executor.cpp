@@ -1,0 +1,439 @@
+// Host time-block loop over the C ABI (class glx::Executor) and its C entry
+// points (glx_exec_*).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <limits>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../errors.h"
+#include "glisser_amd.h"
+#include "glisser_amd_executor.hpp"
+#include "planets.h"
+
+namespace glx {
+
+namespace {
+
+void check(int status, const char* what) {
+    if (status != GLX_OK) throw std::runtime_error(std::string(what) + ": " + glx_last_error());
+}
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct Executor::Impl {
+    ExecutorConfig cfg;
+    host::PlanetIntegrator planets;
+    glx_ctx* ctx = nullptr;
+    uint32_t nb = 0;
+    uint64_t n_total = 0, n_alive = 0;
+    double t = 0.0;
+    int buf = 0;
+    // planet row at time t (positions of bodies 1..nb-1, then h0) and the
+    // encounter radii of the block that starts at t
+    std::vector<double> row_r, row_h0, rh_next;
+    // next block, produced while the kernel runs
+    std::vector<double> log_r, log_v, log_h0;
+    std::vector<EncounterRecord> queue;
+    std::unordered_map<uint32_t, double> deathtime;
+    double planet_ms = 0.0;
+
+    Impl(const ExecutorConfig& c, host::PlanetIntegrator&& p) : cfg(c), planets(std::move(p)) {}
+
+    ~Impl() {
+        if (ctx) glx_destroy(ctx);
+    }
+
+    // Planet block for [t, t + tbsize dt] into log_*, with the radii of its start.
+    void produce_block() {
+        const auto t0 = Clock::now();
+        planets.recalculate_rh();
+        rh_next = planets.rh();
+        const size_t np = nb - 1;
+        log_r.resize((size_t)cfg.tbsize * np * 3);
+        log_v.resize((size_t)cfg.tbsize * np * 3);
+        log_h0.resize((size_t)cfg.tbsize * 3);
+        planets.integrate_block(cfg.tbsize, cfg.dt, log_r.data(), log_v.data(), log_h0.data());
+        planet_ms = ms_since(t0);
+    }
+
+    void upload_block() {
+        check(glx_upload_planet_block(ctx, buf, nb, planets.m().data(), planets.id().data(), rh_next.data(),
+                                      log_r.data(), log_h0.data(), cfg.tbsize),
+              "glx_upload_planet_block");
+    }
+
+    // Records particles [begin, end) flagged during the block that started at
+    // t_block; in resolve mode the encounter class [begin, begin + n_enc)
+    // also goes to the queue with its rollback state.
+    void record_flagged(uint64_t begin, uint64_t end, uint64_t n_enc, double t_block) {
+        const uint64_t n = end - begin;
+        if (n == 0) return;
+        std::vector<uint32_t> id(n), dti(n);
+        std::vector<uint16_t> flags(n);
+        std::vector<double> r, v, r0, v0;
+        if (n_enc) {
+            r.resize(n_enc * 3);
+            v.resize(n_enc * 3);
+            r0.resize(n_enc * 3);
+            v0.resize(n_enc * 3);
+            check(glx_download_particles(ctx, begin, n_enc, r.data(), v.data(), nullptr, nullptr, nullptr, nullptr),
+                  "glx_download_particles");
+            check(glx_download_rollback(ctx, begin, n_enc, r0.data(), v0.data()), "glx_download_rollback");
+        }
+        check(glx_download_particles(ctx, begin, n, nullptr, nullptr, nullptr, flags.data(), id.data(), dti.data()),
+              "glx_download_particles");
+        for (uint64_t k = 0; k < n; k++) {
+            deathtime[id[k]] = t_block + (double)dti[k] * cfg.dt;
+            if (k < n_enc) {
+                EncounterRecord e{};
+                e.id = id[k];
+                e.flags = flags[k];
+                e.step = dti[k];
+                e.t_block = t_block;
+                std::memcpy(e.r0, &r0[3 * k], sizeof e.r0);
+                std::memcpy(e.v0, &v0[3 * k], sizeof e.v0);
+                std::memcpy(e.r, &r[3 * k], sizeof e.r);
+                std::memcpy(e.v, &v[3 * k], sizeof e.v);
+                queue.push_back(e);
+            }
+        }
+    }
+
+    // Stable compaction of [0, n) (kill first unless resolving); returns
+    // the encounter-class size.
+    uint64_t compact(uint64_t n) {
+        check(glx_set_counts(ctx, n, n_total), "glx_set_counts");
+        if (!cfg.resolve_encounters) check(glx_kill_encounters(ctx), "glx_kill_encounters");
+        uint64_t na = 0, ne = 0;
+        check(glx_partition(ctx, cfg.resolve_encounters ? GLX_PART_THREE_WAY : GLX_PART_UNFLAGGED, &na, &ne),
+              "glx_partition");
+        n_alive = na;
+        return ne;
+    }
+};
+
+Executor::Executor(const ExecutorConfig& cfg, uint32_t n_bodies, const double* m, const double* r, const double* v,
+                   const uint32_t* planet_ids, uint64_t n, const double* pr, const double* pv, const uint32_t* ids) {
+    if (n_bodies < 1 || n_bodies > GLX_MAX_BODIES) throw std::runtime_error("Executor: n_bodies outside [1, 16]");
+    if (!m || !r || !v || (n && (!pr || !pv || !ids))) throw std::runtime_error("Executor: null argument");
+    if (cfg.tbsize < 1 || !(cfg.dt > 0.0)) throw std::runtime_error("Executor: tbsize and dt must be positive");
+    std::vector<double> mm(m, m + n_bodies);
+    std::vector<host::Vec3> rr(n_bodies), vv(n_bodies);
+    std::vector<uint32_t> pid(n_bodies);
+    for (uint32_t i = 0; i < n_bodies; i++) {
+        rr[i] = host::Vec3::load(r, i);
+        vv[i] = host::Vec3::load(v, i);
+        pid[i] = planet_ids ? planet_ids[i] : i;
+    }
+    p_ = std::make_unique<Impl>(cfg, host::PlanetIntegrator(std::move(mm), std::move(rr), std::move(vv),
+                                                            std::move(pid), cfg.cull_radius, cfg.rh_factor,
+                                                            cfg.outer_radius));
+    Impl& s = *p_;
+    s.nb = n_bodies;
+    s.n_total = n;
+    s.t = cfg.t0;
+    glx_params gp{};
+    gp.n_particles = n;
+    gp.n_bodies_max = n_bodies;
+    gp.tbsize_max = cfg.tbsize;
+    gp.device = cfg.device;
+    gp.keep_rollback = cfg.resolve_encounters ? 1 : 0;
+    gp.n_planet_buffers = 2;
+    check(glx_create(&gp, &s.ctx), "glx_create");
+    if (n) {
+        std::vector<uint16_t> flags(n, 0);
+        std::vector<uint32_t> dti(n, 0);
+        std::vector<double> a(3 * n, 0.0);
+        check(glx_upload_particles(s.ctx, 0, n, pr, pv, a.data(), flags.data(), ids, dti.data()),
+              "glx_upload_particles");
+    }
+    // planets at t0: positions, h0 and radii (the WHIntegrator constructor,
+    // src/wh.cpp:498-543)
+    s.row_r.resize((size_t)(n_bodies - 1) * 3);
+    for (uint32_t i = 1; i < n_bodies; i++) s.planets.r()[i].store(s.row_r.data(), i - 1);
+    const host::Vec3 h0 = s.planets.h0_initial();
+    s.row_h0 = {h0.x, h0.y, h0.z};
+    check(glx_set_counts(s.ctx, n, n), "glx_set_counts");
+    if (n) {
+        check(glx_init_accel(s.ctx, n_bodies, s.planets.m().data(), s.planets.id().data(), s.planets.rh().data(),
+                             s.row_r.data(), s.row_h0.data(), cfg.outer_radius),
+              "glx_init_accel");
+        if (cfg.resolve_encounters) check(glx_snapshot_rollback(s.ctx), "glx_snapshot_rollback");
+        const uint64_t ne = s.compact(n);
+        s.record_flagged(s.n_alive, n, cfg.resolve_encounters ? ne : 0, cfg.t0);
+    }
+    s.produce_block();
+    s.upload_block();
+}
+
+Executor::~Executor() = default;
+
+void Executor::step() {
+    Impl& s = *p_;
+    const uint64_t prev = s.n_alive;
+    const double t_block = s.t;
+    check(glx_set_counts(s.ctx, prev, s.n_total), "glx_set_counts");
+    if (prev) {
+        if (s.cfg.resolve_encounters) check(glx_snapshot_rollback(s.ctx), "glx_snapshot_rollback");
+        check(glx_run_block(s.ctx, s.buf, s.cfg.dt, s.cfg.outer_radius, s.cfg.form), "glx_run_block");
+    }
+    // the planet row at the end of this block, then the next block (the
+    // host work overlaps the kernel)
+    const size_t np = s.nb - 1;
+    std::memcpy(s.row_r.data(), s.log_r.data() + (size_t)(s.cfg.tbsize - 1) * np * 3, np * 3 * sizeof(double));
+    std::memcpy(s.row_h0.data(), s.log_h0.data() + (size_t)(s.cfg.tbsize - 1) * 3, 3 * sizeof(double));
+    s.produce_block();
+    if (prev) {
+        const uint64_t ne = s.compact(prev);
+        s.record_flagged(s.n_alive, prev, s.cfg.resolve_encounters ? ne : 0, t_block);
+    }
+    s.buf ^= 1;
+    s.upload_block();
+    s.t = t_block + (double)s.cfg.tbsize * s.cfg.dt;
+}
+
+void Executor::run(uint32_t n_blocks) {
+    for (uint32_t k = 0; k < n_blocks; k++) step();
+}
+
+double Executor::time() const { return p_->t; }
+uint64_t Executor::n_alive() const { return p_->n_alive; }
+uint64_t Executor::n_total() const { return p_->n_total; }
+uint64_t Executor::n_queued() const { return p_->queue.size(); }
+
+std::vector<EncounterRecord> Executor::take_encounters() {
+    std::vector<EncounterRecord> out;
+    out.swap(p_->queue);
+    return out;
+}
+
+void Executor::return_particles(uint64_t n, const uint32_t* ids, const double* r, const double* v) {
+    Impl& s = *p_;
+    if (n == 0) return;
+    if (!ids || !r || !v) throw std::runtime_error("return_particles: null argument");
+    const uint64_t begin = s.n_alive, rest = s.n_total - s.n_alive;
+    std::vector<double> rr(rest * 3), vv(rest * 3);
+    std::vector<uint16_t> flags(rest);
+    std::vector<uint32_t> id(rest);
+    check(glx_download_particles(s.ctx, begin, rest, rr.data(), vv.data(), nullptr, flags.data(), id.data(), nullptr),
+          "glx_download_particles");
+    std::unordered_map<uint32_t, uint64_t> slot;
+    for (uint64_t k = 0; k < rest; k++) slot[id[k]] = k;
+    for (uint64_t k = 0; k < n; k++) {
+        auto it = slot.find(ids[k]);
+        if (it == slot.end() || (flags[it->second] & 0xFE) != 0 || flags[it->second] == 0)
+            throw std::runtime_error("return_particles: particle " + std::to_string(ids[k]) +
+                                     " is not waiting in the encounter class");
+        const uint64_t j = it->second;
+        std::memcpy(&rr[3 * j], r + 3 * k, 3 * sizeof(double));
+        std::memcpy(&vv[3 * j], v + 3 * k, 3 * sizeof(double));
+        flags[j] = 0;
+        s.deathtime.erase(ids[k]);
+    }
+    check(glx_upload_particles(s.ctx, begin, rest, rr.data(), vv.data(), nullptr, flags.data(), id.data(), nullptr),
+          "glx_upload_particles");
+    // the returned particles join the unflagged class right after the alive ones
+    check(glx_set_counts(s.ctx, s.n_total, s.n_total), "glx_set_counts");
+    uint64_t na = 0, ne = 0;
+    check(glx_partition(s.ctx, GLX_PART_THREE_WAY, &na, &ne), "glx_partition");
+    if (na != begin + n) throw std::runtime_error("return_particles: compaction count mismatch");
+    check(glx_accel_range(s.ctx, begin, n, s.nb, s.planets.m().data(), s.planets.id().data(), s.rh_next.data(),
+                          s.row_r.data(), s.row_h0.data(), s.cfg.outer_radius),
+          "glx_accel_range");
+    // clear a fresh encounter flag: the next step detects a lasting one
+    std::vector<uint16_t> f(n);
+    check(glx_download_particles(s.ctx, begin, n, nullptr, nullptr, nullptr, f.data(), nullptr, nullptr),
+          "glx_download_particles");
+    std::vector<double> rn(n * 3), vn(n * 3), an(n * 3);
+    std::vector<uint32_t> idn(n);
+    check(glx_download_particles(s.ctx, begin, n, rn.data(), vn.data(), an.data(), nullptr, idn.data(), nullptr),
+          "glx_download_particles");
+    for (auto& x : f)
+        if ((x & 0xFF) == 0x01) x = 0;
+    check(glx_upload_particles(s.ctx, begin, n, rn.data(), vn.data(), an.data(), f.data(), idn.data(), nullptr),
+          "glx_upload_particles");
+    check(glx_set_counts(s.ctx, s.n_total, s.n_total), "glx_set_counts");
+    check(glx_partition(s.ctx, GLX_PART_THREE_WAY, &na, &ne), "glx_partition");
+    s.n_alive = na;
+    check(glx_set_counts(s.ctx, s.n_alive, s.n_total), "glx_set_counts");
+}
+
+ParticleState Executor::download() const {
+    const Impl& s = *p_;
+    ParticleState out;
+    const uint64_t n = s.n_total;
+    out.r.resize(3 * n);
+    out.v.resize(3 * n);
+    out.flags.resize(n);
+    out.id.resize(n);
+    out.deathtime.assign(n, std::numeric_limits<double>::quiet_NaN());
+    check(glx_download_particles(s.ctx, 0, n, out.r.data(), out.v.data(), nullptr, out.flags.data(), out.id.data(),
+                                 nullptr),
+          "glx_download_particles");
+    for (uint64_t k = 0; k < n; k++) {
+        auto it = s.deathtime.find(out.id[k]);
+        if (it != s.deathtime.end()) out.deathtime[k] = it->second;
+    }
+    out.n_alive = s.n_alive;
+    return out;
+}
+
+std::vector<double> Executor::planet_positions() const { return p_->row_r; }
+
+void Executor::last_times(double* kernel_ms, double* partition_ms, double* planet_ms) const {
+    float k = 0.0f, q = 0.0f;
+    check(glx_last_times(p_->ctx, &k, &q), "glx_last_times");
+    if (kernel_ms) *kernel_ms = k;
+    if (partition_ms) *partition_ms = q;
+    if (planet_ms) *planet_ms = p_->planet_ms;
+}
+
+}  // namespace glx
+
+// ---------------------------------------------------------------- C ABI ----
+
+struct glx_exec {
+    std::unique_ptr<glx::Executor> ex;
+};
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return glx::set_error(GLX_ENOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        // keep the status text of a failed C call; the message names it
+        return glx::set_error(GLX_ESTATE, e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int glx_exec_create(const glx_exec_params* p, uint32_t n_bodies, const double* m, const double* r, const double* v,
+                    const uint32_t* planet_id, uint64_t n, const double* pr, const double* pv, const uint32_t* ids,
+                    glx_exec** out) {
+    if (!p || !out) return glx::set_error(GLX_EINVAL, "glx_exec_create: null argument");
+    *out = nullptr;
+    return guarded([&]() -> int {
+        glx::ExecutorConfig c;
+        c.t0 = p->t0;
+        c.dt = p->dt;
+        c.tbsize = p->tbsize;
+        c.outer_radius = p->outer_radius;
+        c.cull_radius = p->cull_radius;
+        c.rh_factor = p->rh_factor;
+        c.resolve_encounters = p->resolve_encounters != 0;
+        c.form = p->form;
+        c.device = p->device;
+        auto e = std::make_unique<glx_exec>();
+        e->ex = std::make_unique<glx::Executor>(c, n_bodies, m, r, v, planet_id, n, pr, pv, ids);
+        *out = e.release();
+        glx::clear_error();
+        return GLX_OK;
+    });
+}
+
+int glx_exec_destroy(glx_exec* e) {
+    delete e;
+    return GLX_OK;
+}
+
+int glx_exec_step(glx_exec* e, uint32_t n_blocks) {
+    if (!e) return glx::set_error(GLX_EINVAL, "null executor");
+    return guarded([&]() -> int {
+        e->ex->run(n_blocks);
+        return GLX_OK;
+    });
+}
+
+int glx_exec_status(glx_exec* e, double* t, uint64_t* n_alive, uint64_t* n_total, uint64_t* n_queued) {
+    if (!e) return glx::set_error(GLX_EINVAL, "null executor");
+    if (t) *t = e->ex->time();
+    if (n_alive) *n_alive = e->ex->n_alive();
+    if (n_total) *n_total = e->ex->n_total();
+    if (n_queued) *n_queued = e->ex->n_queued();
+    return GLX_OK;
+}
+
+int glx_exec_take_encounters(glx_exec* e, uint64_t capacity, glx_encounter* out, uint64_t* n_out) {
+    if (!e || !n_out) return glx::set_error(GLX_EINVAL, "null argument");
+    return guarded([&]() -> int {
+        const uint64_t n = e->ex->n_queued();
+        if (n > capacity || (n && !out)) {
+            *n_out = n;
+            return glx::set_error(GLX_EINVAL, "encounter queue holds " + std::to_string(n) + " records, capacity " +
+                                                  std::to_string(capacity));
+        }
+        auto recs = e->ex->take_encounters();
+        for (uint64_t k = 0; k < n; k++) {
+            const auto& r = recs[k];
+            glx_encounter& o = out[k];
+            o.id = r.id;
+            o.flags = r.flags;
+            o.step = r.step;
+            o.t_block = r.t_block;
+            std::memcpy(o.r0, r.r0, sizeof o.r0);
+            std::memcpy(o.v0, r.v0, sizeof o.v0);
+            std::memcpy(o.r, r.r, sizeof o.r);
+            std::memcpy(o.v, r.v, sizeof o.v);
+        }
+        *n_out = n;
+        return GLX_OK;
+    });
+}
+
+int glx_exec_return(glx_exec* e, uint64_t n, const uint32_t* ids, const double* r, const double* v) {
+    if (!e) return glx::set_error(GLX_EINVAL, "null executor");
+    return guarded([&]() -> int {
+        e->ex->return_particles(n, ids, r, v);
+        return GLX_OK;
+    });
+}
+
+int glx_exec_download(glx_exec* e, double* r, double* v, uint16_t* flags, uint32_t* id, double* deathtime) {
+    if (!e) return glx::set_error(GLX_EINVAL, "null executor");
+    return guarded([&]() -> int {
+        const glx::ParticleState s = e->ex->download();
+        const size_t n = s.id.size();
+        if (r) std::memcpy(r, s.r.data(), 3 * n * sizeof(double));
+        if (v) std::memcpy(v, s.v.data(), 3 * n * sizeof(double));
+        if (flags) std::memcpy(flags, s.flags.data(), n * sizeof(uint16_t));
+        if (id) std::memcpy(id, s.id.data(), n * sizeof(uint32_t));
+        if (deathtime) std::memcpy(deathtime, s.deathtime.data(), n * sizeof(double));
+        return GLX_OK;
+    });
+}
+
+int glx_exec_planets(glx_exec* e, double* r) {
+    if (!e || !r) return glx::set_error(GLX_EINVAL, "null argument");
+    const auto p = e->ex->planet_positions();
+    std::memcpy(r, p.data(), p.size() * sizeof(double));
+    return GLX_OK;
+}
+
+int glx_exec_times(glx_exec* e, double* kernel_ms, double* partition_ms, double* planet_ms) {
+    if (!e) return glx::set_error(GLX_EINVAL, "null executor");
+    return guarded([&]() -> int {
+        e->ex->last_times(kernel_ms, partition_ms, planet_ms);
+        return GLX_OK;
+    });
+}
+
+}  // extern "C"

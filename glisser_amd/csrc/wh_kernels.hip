@@ -147,9 +147,9 @@ hipError_t launch_block(const BlockArgs& A, int form, size_t lds_bytes, hipStrea
 }
 
 hipError_t launch_init_accel(const BlockArgs& A, hipStream_t stream) {
-    if (A.n == 0) return hipSuccess;
+    if (A.n <= A.first) return hipSuccess;
     const uint32_t threads = 256;
-    const uint32_t blocks = (A.n + threads - 1) / threads;
+    const uint32_t blocks = (A.n - A.first + threads - 1) / threads;
     const size_t lds_bytes = (size_t)A.n_bodies * 3 * sizeof(double);
     hipLaunchKernelGGL((init_accel_kernel<0>), dim3(blocks), dim3(threads), lds_bytes, stream, A);
     return hipGetLastError();

@@ -76,6 +76,35 @@ class BatchConfig(C.Structure):
 
 BATCH_MAX_BODIES = 9
 
+
+class ExecParams(C.Structure):
+    """glx_exec_params: configuration of the host time-block loop."""
+    _fields_ = [
+        ("t0", C.c_double),
+        ("dt", C.c_double),
+        ("tbsize", C.c_uint32),
+        ("outer_radius", C.c_double),
+        ("cull_radius", C.c_double),
+        ("rh_factor", C.c_double),
+        ("resolve_encounters", C.c_int32),
+        ("form", C.c_int32),
+        ("device", C.c_int32),
+    ]
+
+
+class EncounterRec(C.Structure):
+    """glx_encounter: one record of the host close-encounter queue."""
+    _fields_ = [
+        ("id", C.c_uint32),
+        ("flags", C.c_uint16),
+        ("step", C.c_uint32),
+        ("t_block", C.c_double),
+        ("r0", C.c_double * 3),
+        ("v0", C.c_double * 3),
+        ("r", C.c_double * 3),
+        ("v", C.c_double * 3),
+    ]
+
 # name -> (restype, argtypes); every symbol declared in include/glisser_amd.h
 SIGNATURES = {
     "glx_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
@@ -112,6 +141,18 @@ SIGNATURES = {
     "glx_batch_tune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BatchConfig), C.c_uint32, C.POINTER(BatchConfig),
                                  C.POINTER(C.c_double)]),
     "glx_batch_sum_steps": (C.c_int, [C.c_void_p, _u64p]),
+    "glx_accel_range": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, _dp, _u32p, _dp, _dp, _dp,
+                                  C.c_double]),
+    "glx_exec_create": (C.c_int, [C.POINTER(ExecParams), C.c_uint32, _dp, _dp, _dp, _u32p, C.c_uint64, _dp, _dp,
+                                  _u32p, C.POINTER(C.c_void_p)]),
+    "glx_exec_destroy": (C.c_int, [C.c_void_p]),
+    "glx_exec_step": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "glx_exec_status": (C.c_int, [C.c_void_p, _dp, _u64p, _u64p, _u64p]),
+    "glx_exec_take_encounters": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(EncounterRec), _u64p]),
+    "glx_exec_return": (C.c_int, [C.c_void_p, C.c_uint64, _u32p, _dp, _dp]),
+    "glx_exec_download": (C.c_int, [C.c_void_p, _dp, _dp, _u16p, _u32p, _dp]),
+    "glx_exec_planets": (C.c_int, [C.c_void_p, _dp]),
+    "glx_exec_times": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "glx_planets_create": (C.c_int, [C.c_uint32, _dp, _dp, _dp, _u32p, C.c_double, C.c_double, C.c_double,
                                      C.POINTER(C.c_void_p)]),
     "glx_planets_destroy": (C.c_int, [C.c_void_p]),
@@ -494,6 +535,106 @@ class Batch:
         out = C.c_uint64(0)
         self._check(self.lib.glx_batch_sum_steps(self.h, C.byref(out)))
         return int(out.value)
+
+
+class Executor:
+    """Host time-block loop (glx_exec_*): planets from the host Jacobi WH
+    producer, particles on the device, optional encounter hand-off."""
+
+    def __init__(self, m, r, v, pr, pv, ids=None, planet_ids=None, t0: float = 0.0, dt: float = 180.0,
+                 tbsize: int = 2048, outer_radius: float = 300.0, cull_radius: float = 0.5, rh_factor: float = 0.0,
+                 resolve_encounters: bool = False, form: str | int = "fp64", device: int = 0):
+        self.lib = load()
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        self.nb = len(m)
+        r = _vec3(r, self.nb)
+        v = _vec3(v, self.nb)
+        n = len(pr)
+        pr = _vec3(pr, n)
+        pv = _vec3(pv, n)
+        ids = np.arange(n, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+        pid = None if planet_ids is None else np.ascontiguousarray(planet_ids, dtype=np.uint32)
+        f = FORMS[form] if isinstance(form, str) else int(form)
+        p = ExecParams(t0, dt, tbsize, outer_radius, cull_radius, rh_factor, int(resolve_encounters), f, device)
+        h = C.c_void_p()
+        rc = self.lib.glx_exec_create(C.byref(p), self.nb, _ptr(m, _dp), _ptr(r, _dp), _ptr(v, _dp),
+                                      _ptr(pid, _u32p), n, _ptr(pr, _dp), _ptr(pv, _dp), _ptr(ids, _u32p),
+                                      C.byref(h))
+        if rc != 0:
+            raise GlxError(rc, self.lib.glx_last_error().decode())
+        self.h = h
+        self.n = n
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise GlxError(rc, self.lib.glx_last_error().decode())
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.glx_exec_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n_blocks: int = 1) -> None:
+        self._check(self.lib.glx_exec_step(self.h, n_blocks))
+
+    def status(self) -> dict:
+        t = C.c_double()
+        a, tot, q = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._check(self.lib.glx_exec_status(self.h, C.byref(t), C.byref(a), C.byref(tot), C.byref(q)))
+        return {"t": t.value, "n_alive": a.value, "n_total": tot.value, "n_queued": q.value}
+
+    def take_encounters(self) -> dict:
+        """Queued records as arrays: id, flags, step, t_block, r0, v0, r, v."""
+        n = self.status()["n_queued"]
+        buf = (EncounterRec * max(n, 1))()
+        got = C.c_uint64()
+        self._check(self.lib.glx_exec_take_encounters(self.h, n, buf, C.byref(got)))
+        recs = buf[:got.value]
+        out = {"id": np.array([x.id for x in recs], np.uint32),
+               "flags": np.array([x.flags for x in recs], np.uint16),
+               "step": np.array([x.step for x in recs], np.uint32),
+               "t_block": np.array([x.t_block for x in recs])}
+        for k in ("r0", "v0", "r", "v"):
+            out[k] = np.array([list(getattr(x, k)) for x in recs]).reshape(-1, 3)
+        return out
+
+    def return_particles(self, ids, r, v) -> None:
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        r = _vec3(r, len(ids))
+        v = _vec3(v, len(ids))
+        self._check(self.lib.glx_exec_return(self.h, len(ids), _ptr(ids, _u32p), _ptr(r, _dp), _ptr(v, _dp)))
+
+    def download(self) -> dict:
+        n = self.n
+        out = {"r": np.empty((n, 3)), "v": np.empty((n, 3)), "flags": np.empty(n, np.uint16),
+               "id": np.empty(n, np.uint32), "deathtime": np.empty(n)}
+        self._check(self.lib.glx_exec_download(self.h, _ptr(out["r"], _dp), _ptr(out["v"], _dp),
+                                               _ptr(out["flags"], _u16p), _ptr(out["id"], _u32p),
+                                               _ptr(out["deathtime"], _dp)))
+        out.update(self.status())
+        return out
+
+    def planet_positions(self) -> np.ndarray:
+        out = np.empty((self.nb - 1, 3))
+        self._check(self.lib.glx_exec_planets(self.h, _ptr(out, _dp)))
+        return out
+
+    def times(self) -> tuple[float, float, float]:
+        k, q, p = C.c_double(), C.c_double(), C.c_double()
+        self._check(self.lib.glx_exec_times(self.h, C.byref(k), C.byref(q), C.byref(p)))
+        return k.value, q.value, p.value
 
 
 def load_h0(m, r_log) -> np.ndarray:

@@ -156,6 +156,15 @@ int glx_run_blocks(glx_ctx* ctx, uint32_t n_blocks, int first_buf, double dt, do
 int glx_init_accel(glx_ctx* ctx, uint32_t n_bodies, const double* m, const uint32_t* id, const double* rh,
                    const double* planet_r, const double* h0, double outer_radius);
 
+/* glx_init_accel over particles [begin, begin+n) instead of [0, n_alive):
+ * accelerations and flags of particles handed back after an encounter was
+ * resolved on the host, from the planet row of the current time (the
+ * helio_acc_particles call of Executor::handle_encounters,
+ * src/executor.cu:558-567), synchronous. */
+int glx_accel_range(glx_ctx* ctx, uint64_t begin, uint64_t n, uint32_t n_bodies, const double* m,
+                    const uint32_t* id, const double* rh, const double* planet_r, const double* h0,
+                    double outer_radius);
+
 /* flags & 0x01 -> (flags & 0xFF00) | 0x80 over [0, n_alive), async.
  * Replaces KillEncounterKernel (src/executor.cu:63-78,656-660). */
 int glx_kill_encounters(glx_ctx* ctx);
@@ -275,6 +284,60 @@ int glx_batch_tune(glx_batch* b, int form, const glx_batch_config* candidates, u
 /* Sum of deathtime_index over every particle of the batch (synchronous):
  * the particle-steps done since the batch started. */
 int glx_batch_sum_steps(glx_batch* b, uint64_t* total);
+
+/* ---------------------------------------------------------------------
+ * Host time-block loop (C entry points of the C++ class glx::Executor in
+ * glisser_amd_executor.hpp): Executor::init / loop / resync2 / the
+ * encounter hand-off of the reference (src/executor.cu:128-877), driving the
+ * entry points above.  Planets come from the host Jacobi WH producer.
+ * ------------------------------------------------------------------- */
+
+typedef struct glx_exec glx_exec;
+
+typedef struct glx_exec_params {
+    double t0;
+    double dt;                  /* Time-Step */
+    uint32_t tbsize;            /* Time-Block-Size */
+    double outer_radius;        /* Outer-Limit */
+    double cull_radius;         /* Cull-Radius */
+    double rh_factor;           /* Encounter-RH-Factor */
+    int32_t resolve_encounters; /* Resolve-Encounters: hand encounter particles to the host queue */
+    int32_t form;               /* glx_form */
+    int32_t device;
+} glx_exec_params;
+
+/* One record of the host close-encounter queue. */
+typedef struct glx_encounter {
+    uint32_t id;
+    uint16_t flags;     /* 0x01, planet id in the high byte (0: the Sun) */
+    uint32_t step;      /* steps of the block completed when flagged */
+    double t_block;     /* time at the start of that block */
+    double r0[3], v0[3];  /* state at the start of the block (rollback copy) */
+    double r[3], v[3];    /* state at the flagging step */
+} glx_encounter;
+
+/* Create the loop: planets (n_bodies GM values, heliocentric r, v), n
+ * particles (heliocentric pr, pv, ids).  Uploads, computes the initial
+ * accelerations and flags, compacts, produces the first planet block. */
+int glx_exec_create(const glx_exec_params* p, uint32_t n_bodies, const double* m, const double* r, const double* v,
+                    const uint32_t* planet_id, uint64_t n, const double* pr, const double* pv, const uint32_t* ids,
+                    glx_exec** out);
+int glx_exec_destroy(glx_exec* e);
+/* n_blocks time-blocks (Executor::loop + resync2 each). */
+int glx_exec_step(glx_exec* e, uint32_t n_blocks);
+int glx_exec_status(glx_exec* e, double* t, uint64_t* n_alive, uint64_t* n_total, uint64_t* n_queued);
+/* Move every queued encounter record into out (capacity entries); GLX_EINVAL
+ * with *n_out = queue length if it does not fit. */
+int glx_exec_take_encounters(glx_exec* e, uint64_t capacity, glx_encounter* out, uint64_t* n_out);
+/* Hand resolved particles back at the loop's current time (r, v: n x 3). */
+int glx_exec_return(glx_exec* e, uint64_t n, const uint32_t* ids, const double* r, const double* v);
+/* Every particle slot (n_total): r, v (n x 3), flags, id, deathtime (time of
+ * the flagging step, NaN if never flagged); any output may be NULL. */
+int glx_exec_download(glx_exec* e, double* r, double* v, uint16_t* flags, uint32_t* id, double* deathtime);
+/* Heliocentric planet positions ((n_bodies - 1) x 3) at the current time. */
+int glx_exec_planets(glx_exec* e, double* r);
+/* Device time of the last kernel and compaction, host time of the last planet block (ms). */
+int glx_exec_times(glx_exec* e, double* kernel_ms, double* partition_ms, double* planet_ms);
 
 /* ---------------------------------------------------------------------
  * Host-side producer of the kernel's planet input and orbital conversions.

@@ -1,0 +1,101 @@
+// glisser_amd_executor.hpp -- C++ host API of the time-block loop.
+//
+// glx::Executor is the C++ counterpart of the reference's Executor
+// (src/executor.cu:598-877) driving the C ABI of glisser_amd.h: per
+// time-block it launches the particle kernel on the planet block produced
+// during the previous block, produces the next planet block on the host
+// while the kernel runs (planet Jacobi WH, src/wh.cpp:584-595), then
+// compacts the particles and records the newly flagged ones.  With
+// resolve_encounters, particles flagged for a planet encounter are not
+// killed: the stable three-way compaction keeps them alive, their state at
+// the start of the block (the rollback copy) and at the flagging step go to
+// the host close-encounter queue, and a resolver hands them back with
+// return_particles() -- the hand-off of Executor::resync2 /
+// handle_encounters (src/executor.cu:470-596,766-877); the resolver itself
+// (SWIFT in the reference, src/swift.cpp) is the caller's.
+//
+// Errors throw std::runtime_error, as the reference host code does.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "glisser_amd.h"
+
+namespace glx {
+
+struct ExecutorConfig {
+    double t0 = 0.0;
+    double dt = 180.0;                // Time-Step
+    uint32_t tbsize = 2048;           // Time-Block-Size
+    double outer_radius = 300.0;      // Outer-Limit
+    double cull_radius = 0.5;         // Cull-Radius
+    double rh_factor = 0.0;           // Encounter-RH-Factor
+    bool resolve_encounters = false;  // Resolve-Encounters
+    int form = GLX_FORM_FP64;         // kernel arithmetic
+    int device = 0;
+};
+
+// One particle handed to the host close-encounter queue.
+struct EncounterRecord {
+    uint32_t id;
+    uint16_t flags;    // 0x01 with the planet id in the high byte (0: the Sun)
+    uint32_t step;     // steps of the block completed when it was flagged
+    double t_block;    // time at the start of that block
+    double r0[3], v0[3];  // state at the start of the block (rollback copy)
+    double r[3], v[3];    // state at the flagging step
+};
+
+// Host copy of every particle slot (r, v: n x 3).
+struct ParticleState {
+    std::vector<double> r, v, deathtime;  // deathtime: time of the flagging step (NaN if never flagged)
+    std::vector<uint16_t> flags;
+    std::vector<uint32_t> id;
+    uint64_t n_alive = 0;
+};
+
+class Executor {
+public:
+    // Planets: n_bodies GM values m (m[0] the Sun), heliocentric r, v
+    // (n_bodies x 3), ids (NULL: 0..n-1).  Particles: n heliocentric r, v
+    // and ids.  Uploads everything, computes the initial accelerations and
+    // flags, compacts, and produces the first planet block.
+    Executor(const ExecutorConfig& cfg, uint32_t n_bodies, const double* m, const double* r, const double* v,
+             const uint32_t* planet_ids, uint64_t n, const double* pr, const double* pv, const uint32_t* ids);
+    ~Executor();
+    Executor(const Executor&) = delete;
+    Executor& operator=(const Executor&) = delete;
+
+    // One time-block (Executor::loop + resync2).
+    void step();
+    void run(uint32_t n_blocks);
+
+    double time() const;
+    uint64_t n_alive() const;
+    uint64_t n_total() const;
+    uint64_t n_queued() const;  // records waiting in the encounter queue
+
+    // Take every queued record (the queue is emptied).
+    std::vector<EncounterRecord> take_encounters();
+    // Hand resolved particles back at the current time(): heliocentric r, v
+    // (n x 3) of particles previously taken from the queue.  Their
+    // accelerations are recomputed from the planets at time() and an
+    // encounter flag is cleared (the next step re-detects a lasting one),
+    // as handle_encounters does (src/executor.cu:558-578).
+    void return_particles(uint64_t n, const uint32_t* ids, const double* r, const double* v);
+
+    ParticleState download() const;
+    // Heliocentric planet positions (n_bodies - 1 x 3) at time().
+    std::vector<double> planet_positions() const;
+
+    // Device time of the last kernel and compaction, host time of the last
+    // planet block (ms).
+    void last_times(double* kernel_ms, double* partition_ms, double* planet_ms) const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> p_;
+};
+
+}  // namespace glx
