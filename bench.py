@@ -241,6 +241,75 @@ def run_batch(forms, n_systems: int, launches: int, seed: int) -> dict:
     return out
 
 
+ENC_PARTICLES = 1_000_000
+ENC_BLOCKS = 3
+
+
+def run_encounter(forms, n_particles: int, n_blocks: int, seed: int) -> dict:
+    """Encounter hand-off on the three-body setup: Sun + one planet of 1e-3
+    Msun on a circular orbit at 30 au, particles a ~ U[25, 40] au, e < 0.3,
+    i < 10 deg; encounter radius 3.5 Hill radii; Resolve-Encounters on, so
+    every block is rollback snapshot -> kernel -> stable three-way compaction
+    -> encounter records (block-start and flagging states) to the host queue,
+    with the next planet block produced on the host meanwhile (glx::Executor).
+    Timed: n_blocks blocks after one warm-up block.  Also reports the
+    Jacobi-constant drift of the particles never flagged."""
+    from glisser_amd import native, systems
+
+    s = systems.three_body(n_particles, seed=seed, a_range=(25.0, 40.0), e_hi=0.3, i_hi_deg=10.0)
+    a_p = 30.0
+    out = {"particles": n_particles, "blocks": n_blocks, "tbsize": TBSIZE, "dt_days": DT,
+           "encounter_radius_hill": 3.5, "forms": {}}
+
+    def jacobi(r, v, rp, vp):
+        mu_s, mu_p = s.m
+        M = mu_s + mu_p
+        n = np.sqrt(M / a_p**3)
+        x = r - mu_p * rp / M
+        u = v - mu_p * vp / M
+        return (2 * (mu_s / np.linalg.norm(r, axis=1) + mu_p / np.linalg.norm(r - rp, axis=1))
+                + 2 * n * (x[:, 0] * u[:, 1] - x[:, 1] * u[:, 0]) - np.sum(u * u, axis=1))
+
+    c0 = jacobi(s.pr, s.pv, s.r[1], s.v[1])
+    for form in forms:
+        with native.Executor(s.m, s.r, s.v, s.pr, s.pv, s.ids, dt=DT, tbsize=TBSIZE, rh_factor=3.5,
+                             resolve_encounters=True, form=form) as ex:
+            ex.step(1)
+            queued0 = len(ex.take_encounters()["id"])
+            n0 = ex.status()["n_alive"]
+            t0 = time.perf_counter()
+            handed = 0
+            psteps = 0
+            other_flagged = 0
+            for _ in range(n_blocks):
+                alive = ex.status()["n_alive"]
+                ex.step(1)
+                rec = ex.take_encounters()
+                handed += len(rec["id"])
+                alive_now = ex.status()["n_alive"]
+                # particle-steps: full blocks of the particles still unflagged plus the partial
+                # blocks of those handed to the queue; particles flagged otherwise (outer limit,
+                # none expected in this swarm) are counted as zero steps
+                psteps += alive_now * TBSIZE + int(rec["step"].astype(np.int64).sum())
+                other_flagged += int(alive - alive_now - len(rec["id"]))
+            el = time.perf_counter() - t0
+            kernel_ms, part_ms, planet_ms = ex.times()
+            d = ex.download()
+            rp = ex.planet_positions()[0]
+        vp = np.cross([0.0, 0.0, 1.0], rp) * np.sqrt((s.m[0] + s.m[1]) / a_p**3)
+        order = np.argsort(d["id"])
+        ok = d["flags"][order] == 0
+        c1 = jacobi(d["r"][order], d["v"][order], rp, vp)
+        drift = np.abs(c1 - c0)[ok] / np.abs(c0[ok])
+        out["forms"][form] = {
+            "value": psteps / el, "unit": "particle-steps/s", "ms_per_block": el / n_blocks * 1e3,
+            "kernel_ms_last_block": kernel_ms, "partition_ms_last_block": part_ms, "other_flagged": other_flagged,
+            "planet_block_host_ms": planet_ms, "alive_start": int(n0), "alive_end": int(d["n_alive"]),
+            "handed_to_host": handed, "handed_warmup_block": queued0,
+            "jacobi_drift_max": float(drift.max()), "jacobi_drift_median": float(np.median(drift))}
+    return out
+
+
 def cpu_baseline(inputs, sample_particles: int):
     """Reference CPU path (oracle/_ref: the reference's own src/wh.cpp built
     from its sources) or, where that library is absent, the oracle
@@ -283,6 +352,8 @@ def main(argv=None):
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-system workload")
     ap.add_argument("--batch-systems", type=int, default=BATCH_SYSTEMS)
     ap.add_argument("--batch-launches", type=int, default=BATCH_LAUNCHES)
+    ap.add_argument("--no-encounter", action="store_true", help="skip the encounter hand-off workload")
+    ap.add_argument("--encounter-particles", type=int, default=ENC_PARTICLES)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -382,6 +453,8 @@ def main(argv=None):
             **{f: element_differences(inputs[0].m[0], ref, subset_by_id(finals[f], ref["id"])) for f in finals}}
     if not args.no_batch:
         line["batch"] = run_batch(forms, args.batch_systems, args.batch_launches, args.seed + 2)
+    if not args.no_encounter:
+        line["encounter"] = run_encounter(forms, args.encounter_particles, ENC_BLOCKS, args.seed + 3)
     if rank == 0 and not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(inputs, args.cpu_sample)
     if rank == 0:

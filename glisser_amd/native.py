@@ -76,6 +76,12 @@ class BatchConfig(C.Structure):
 
 BATCH_MAX_BODIES = 9
 
+# numpy view of glx_encounter records (layout of EncounterRec below)
+_ENC_DTYPE = np.dtype({"names": ["id", "flags", "step", "t_block", "r0", "v0", "r", "v"],
+                       "formats": [np.uint32, np.uint16, np.uint32, np.float64, (np.float64, 3), (np.float64, 3),
+                                   (np.float64, 3), (np.float64, 3)],
+                       "offsets": [0, 4, 8, 16, 24, 48, 72, 96], "itemsize": 120})
+
 
 class ExecParams(C.Structure):
     """glx_exec_params: configuration of the host time-block loop."""
@@ -601,13 +607,10 @@ class Executor:
         buf = (EncounterRec * max(n, 1))()
         got = C.c_uint64()
         self._check(self.lib.glx_exec_take_encounters(self.h, n, buf, C.byref(got)))
-        recs = buf[:got.value]
-        out = {"id": np.array([x.id for x in recs], np.uint32),
-               "flags": np.array([x.flags for x in recs], np.uint16),
-               "step": np.array([x.step for x in recs], np.uint32),
-               "t_block": np.array([x.t_block for x in recs])}
+        a = np.frombuffer(buf, dtype=_ENC_DTYPE, count=got.value)
+        out = {k: a[k].copy() for k in ("id", "flags", "step", "t_block")}
         for k in ("r0", "v0", "r", "v"):
-            out[k] = np.array([list(getattr(x, k)) for x in recs]).reshape(-1, 3)
+            out[k] = a[k].reshape(-1, 3).copy()
         return out
 
     def return_particles(self, ids, r, v) -> None:
