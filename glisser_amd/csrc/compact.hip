@@ -195,6 +195,30 @@ __global__ __launch_bounds__(256) void snapshot_kernel(ParticleSet s, RollbackSe
     }
 }
 
+// Particles [begin, begin + n) as host close-encounter records: state at the
+// start of the block from the rollback copy (or the current state when no
+// copy is kept), state now, flags, id and step count.
+__global__ __launch_bounds__(256) void pack_encounters_kernel(ParticleSet s, RollbackSet r, int has_rb,
+                                                              uint32_t begin, uint32_t n, double t_block,
+                                                              glx_encounter* out) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t i = begin + k;
+        glx_encounter e;
+        e.id = s.id[i];
+        e.flags = s.flags[i];
+        e.step = s.dti[i];
+        e.t_block = t_block;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            e.r[c] = s.d[c][i];
+            e.v[c] = s.d[3 + c][i];
+            e.r0[c] = has_rb ? r.d[c][i] : e.r[c];
+            e.v0[c] = has_rb ? r.d[3 + c][i] : e.v[c];
+        }
+        out[k] = e;
+    }
+}
+
 // One workgroup sums dti[0, n) into *out (no atomics, nothing to clear).
 __global__ __launch_bounds__(1024) void sum_steps_kernel(const uint32_t* __restrict__ dti, uint32_t n,
                                                          unsigned long long* out) {
@@ -267,6 +291,15 @@ hipError_t launch_kill_encounters(uint16_t* flags, uint32_t n, hipStream_t strea
 hipError_t launch_snapshot(ParticleSet cur, RollbackSet rb, uint32_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(snapshot_kernel, dim3(grid_for(n)), dim3(kThreads), 0, stream, cur, rb, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_encounters(ParticleSet cur, const RollbackSet* rb, uint32_t begin, uint32_t n, double t_block,
+                                  glx_encounter* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const RollbackSet none{};
+    hipLaunchKernelGGL(pack_encounters_kernel, dim3(grid_for(n)), dim3(kThreads), 0, stream, cur, rb ? *rb : none,
+                       rb ? 1 : 0, begin, n, t_block, out);
     return hipGetLastError();
 }
 

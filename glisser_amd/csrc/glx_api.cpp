@@ -112,6 +112,9 @@ struct glx_ctx {
     uint64_t n_alive = 0, n_total = 0;
     History hist;
     double hist_t0 = 0, hist_t1 = 0;  // first and last history times (host copy)
+    glx_encounter* enc_dev = nullptr;  // encounter-record staging (grown on demand)
+    glx_encounter* enc_pin = nullptr;
+    uint64_t enc_cap = 0;
     std::vector<void*> allocs;
 };
 
@@ -153,6 +156,8 @@ void release(glx_ctx* c) {
     for (void* p : c->allocs) (void)hipFree(p);
     for (double* p : {c->hist.t, c->hist.r, c->hist.v, c->hist.q_times, c->hist.q_rows})
         if (p) (void)hipFree(p);
+    if (c->enc_dev) (void)hipFree(c->enc_dev);
+    if (c->enc_pin) (void)hipHostFree(c->enc_pin);
     for (auto& b : c->pb) {
         if (b.pin_log) (void)hipHostFree(b.pin_log);
         if (b.uploaded) (void)hipEventDestroy(b.uploaded);
@@ -705,6 +710,33 @@ int glx_accel_range(glx_ctx* c, uint64_t begin, uint64_t n, uint32_t n_bodies, c
     if (!c) return fail(GLX_EINVAL, "null context");
     if (int rc = check_range(c, begin, n)) return rc;
     return accel_range(c, begin, begin + n, n_bodies, m, id, rh, planet_r, h0, outer_radius);
+}
+
+int glx_pack_encounters(glx_ctx* c, uint64_t begin, uint64_t n, double t_block, glx_encounter* out) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_range(c, begin, n)) return rc;
+    if (n == 0) return GLX_OK;
+    if (!out) return fail(GLX_EINVAL, "null output");
+    GLX_HIP(hipSetDevice(c->device));
+    if (n > c->enc_cap) {
+        GLX_HIP(hipStreamSynchronize(c->main_s));
+        if (c->enc_dev) (void)hipFree(c->enc_dev);
+        if (c->enc_pin) (void)hipHostFree(c->enc_pin);
+        c->enc_dev = nullptr;
+        c->enc_pin = nullptr;
+        c->enc_cap = 0;
+        const uint64_t cap = std::max<uint64_t>(n, 4096);
+        if (hipMalloc((void**)&c->enc_dev, cap * sizeof(glx_encounter)) != hipSuccess ||
+            hipHostMalloc((void**)&c->enc_pin, cap * sizeof(glx_encounter), hipHostMallocDefault) != hipSuccess)
+            return fail(GLX_ENOMEM, "encounter staging of %llu records failed", (unsigned long long)cap);
+        c->enc_cap = cap;
+    }
+    GLX_HIP(glx::launch_pack_encounters(c->cur, c->keep_rb ? &c->rb : nullptr, (uint32_t)begin, (uint32_t)n, t_block,
+                                        c->enc_dev, c->main_s));
+    GLX_HIP(hipMemcpyAsync(c->enc_pin, c->enc_dev, n * sizeof(glx_encounter), hipMemcpyDeviceToHost, c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    std::memcpy(out, c->enc_pin, n * sizeof(glx_encounter));
+    return GLX_OK;
 }
 
 int glx_kill_encounters(glx_ctx* c) {

@@ -1,5 +1,6 @@
 // Host time-block loop over the C ABI (class glx::Executor) and its C entry
 // points (glx_exec_*).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -46,7 +47,28 @@ struct Executor::Impl {
     // next block, produced while the kernel runs
     std::vector<double> log_r, log_v, log_h0;
     std::vector<EncounterRecord> queue;
-    std::unordered_map<uint32_t, double> deathtime;
+    // time of the flagging step by particle id: a flat table when the ids
+    // are dense (max id < 4 n), else a hash map
+    std::vector<double> deathtime_flat;
+    std::unordered_map<uint32_t, double> deathtime_map;
+
+    void set_deathtime(uint32_t id, double t) {
+        if (!deathtime_flat.empty())
+            deathtime_flat[id] = t;
+        else
+            deathtime_map[id] = t;
+    }
+    void clear_deathtime(uint32_t id) {
+        if (!deathtime_flat.empty())
+            deathtime_flat[id] = std::numeric_limits<double>::quiet_NaN();
+        else
+            deathtime_map.erase(id);
+    }
+    double get_deathtime(uint32_t id) const {
+        if (!deathtime_flat.empty()) return deathtime_flat[id];
+        auto it = deathtime_map.find(id);
+        return it == deathtime_map.end() ? std::numeric_limits<double>::quiet_NaN() : it->second;
+    }
     double planet_ms = 0.0;
 
     Impl(const ExecutorConfig& c, host::PlanetIntegrator&& p) : cfg(c), planets(std::move(p)) {}
@@ -80,35 +102,10 @@ struct Executor::Impl {
     void record_flagged(uint64_t begin, uint64_t end, uint64_t n_enc, double t_block) {
         const uint64_t n = end - begin;
         if (n == 0) return;
-        std::vector<uint32_t> id(n), dti(n);
-        std::vector<uint16_t> flags(n);
-        std::vector<double> r, v, r0, v0;
-        if (n_enc) {
-            r.resize(n_enc * 3);
-            v.resize(n_enc * 3);
-            r0.resize(n_enc * 3);
-            v0.resize(n_enc * 3);
-            check(glx_download_particles(ctx, begin, n_enc, r.data(), v.data(), nullptr, nullptr, nullptr, nullptr),
-                  "glx_download_particles");
-            check(glx_download_rollback(ctx, begin, n_enc, r0.data(), v0.data()), "glx_download_rollback");
-        }
-        check(glx_download_particles(ctx, begin, n, nullptr, nullptr, nullptr, flags.data(), id.data(), dti.data()),
-              "glx_download_particles");
-        for (uint64_t k = 0; k < n; k++) {
-            deathtime[id[k]] = t_block + (double)dti[k] * cfg.dt;
-            if (k < n_enc) {
-                EncounterRecord e{};
-                e.id = id[k];
-                e.flags = flags[k];
-                e.step = dti[k];
-                e.t_block = t_block;
-                std::memcpy(e.r0, &r0[3 * k], sizeof e.r0);
-                std::memcpy(e.v0, &v0[3 * k], sizeof e.v0);
-                std::memcpy(e.r, &r[3 * k], sizeof e.r);
-                std::memcpy(e.v, &v[3 * k], sizeof e.v);
-                queue.push_back(e);
-            }
-        }
+        std::vector<glx_encounter> rec(n);
+        check(glx_pack_encounters(ctx, begin, n, t_block, rec.data()), "glx_pack_encounters");
+        for (uint64_t k = 0; k < n; k++) set_deathtime(rec[k].id, t_block + (double)rec[k].step * cfg.dt);
+        queue.insert(queue.end(), rec.begin(), rec.begin() + n_enc);
     }
 
     // Stable compaction of [0, n) (kill first unless resolving); returns
@@ -152,6 +149,10 @@ Executor::Executor(const ExecutorConfig& cfg, uint32_t n_bodies, const double* m
     gp.keep_rollback = cfg.resolve_encounters ? 1 : 0;
     gp.n_planet_buffers = 2;
     check(glx_create(&gp, &s.ctx), "glx_create");
+    uint32_t max_id = 0;
+    for (uint64_t k = 0; k < n; k++) max_id = std::max(max_id, ids[k]);
+    if (n && (uint64_t)max_id < 4 * n)
+        s.deathtime_flat.assign((size_t)max_id + 1, std::numeric_limits<double>::quiet_NaN());
     if (n) {
         std::vector<uint16_t> flags(n, 0);
         std::vector<uint32_t> dti(n, 0);
@@ -240,7 +241,7 @@ void Executor::return_particles(uint64_t n, const uint32_t* ids, const double* r
         std::memcpy(&rr[3 * j], r + 3 * k, 3 * sizeof(double));
         std::memcpy(&vv[3 * j], v + 3 * k, 3 * sizeof(double));
         flags[j] = 0;
-        s.deathtime.erase(ids[k]);
+        s.clear_deathtime(ids[k]);
     }
     check(glx_upload_particles(s.ctx, begin, rest, rr.data(), vv.data(), nullptr, flags.data(), id.data(), nullptr),
           "glx_upload_particles");
@@ -282,10 +283,7 @@ ParticleState Executor::download() const {
     check(glx_download_particles(s.ctx, 0, n, out.r.data(), out.v.data(), nullptr, out.flags.data(), out.id.data(),
                                  nullptr),
           "glx_download_particles");
-    for (uint64_t k = 0; k < n; k++) {
-        auto it = s.deathtime.find(out.id[k]);
-        if (it != s.deathtime.end()) out.deathtime[k] = it->second;
-    }
+    for (uint64_t k = 0; k < n; k++) out.deathtime[k] = s.get_deathtime(out.id[k]);
     out.n_alive = s.n_alive;
     return out;
 }
@@ -382,18 +380,7 @@ int glx_exec_take_encounters(glx_exec* e, uint64_t capacity, glx_encounter* out,
                                                   std::to_string(capacity));
         }
         auto recs = e->ex->take_encounters();
-        for (uint64_t k = 0; k < n; k++) {
-            const auto& r = recs[k];
-            glx_encounter& o = out[k];
-            o.id = r.id;
-            o.flags = r.flags;
-            o.step = r.step;
-            o.t_block = r.t_block;
-            std::memcpy(o.r0, r.r0, sizeof o.r0);
-            std::memcpy(o.v0, r.v0, sizeof o.v0);
-            std::memcpy(o.r, r.r, sizeof o.r);
-            std::memcpy(o.v, r.v, sizeof o.v);
-        }
+        if (n) std::memcpy(out, recs.data(), n * sizeof(glx_encounter));
         *n_out = n;
         return GLX_OK;
     });

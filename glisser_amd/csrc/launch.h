@@ -43,6 +43,9 @@ hipError_t launch_partition(ParticleSet cur, ParticleSet tmp, const RollbackSet*
 hipError_t launch_kill_encounters(uint16_t* flags, uint32_t n, hipStream_t stream);
 hipError_t launch_snapshot(ParticleSet cur, RollbackSet rb, uint32_t n, hipStream_t stream);
 hipError_t launch_sum_steps(const uint32_t* dti, uint32_t n, unsigned long long* out, hipStream_t stream);
+// Host close-encounter records of particles [begin, begin + n); rb may be null.
+hipError_t launch_pack_encounters(ParticleSet cur, const RollbackSet* rb, uint32_t begin, uint32_t n, double t_block,
+                                  glx_encounter* out, hipStream_t stream);
 
 // A sampled planet history on the device: times t[n_samples] (strictly
 // increasing) and heliocentric r, v of bodies 1..n_bodies-1 as

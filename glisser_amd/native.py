@@ -147,6 +147,7 @@ SIGNATURES = {
     "glx_batch_tune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BatchConfig), C.c_uint32, C.POINTER(BatchConfig),
                                  C.POINTER(C.c_double)]),
     "glx_batch_sum_steps": (C.c_int, [C.c_void_p, _u64p]),
+    "glx_pack_encounters": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double, C.POINTER(EncounterRec)]),
     "glx_accel_range": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, _dp, _u32p, _dp, _dp, _dp,
                                   C.c_double]),
     "glx_exec_create": (C.c_int, [C.POINTER(ExecParams), C.c_uint32, _dp, _dp, _dp, _u32p, C.c_uint64, _dp, _dp,
@@ -604,10 +605,11 @@ class Executor:
     def take_encounters(self) -> dict:
         """Queued records as arrays: id, flags, step, t_block, r0, v0, r, v."""
         n = self.status()["n_queued"]
-        buf = (EncounterRec * max(n, 1))()
+        a = np.empty(max(n, 1), dtype=_ENC_DTYPE)
         got = C.c_uint64()
-        self._check(self.lib.glx_exec_take_encounters(self.h, n, buf, C.byref(got)))
-        a = np.frombuffer(buf, dtype=_ENC_DTYPE, count=got.value)
+        self._check(self.lib.glx_exec_take_encounters(self.h, n, a.ctypes.data_as(C.POINTER(EncounterRec)),
+                                                      C.byref(got)))
+        a = a[:got.value]
         out = {k: a[k].copy() for k in ("id", "flags", "step", "t_block")}
         for k in ("r0", "v0", "r", "v"):
             out[k] = a[k].reshape(-1, 3).copy()

@@ -61,6 +61,17 @@ typedef struct glx_params {
     uint32_t n_planet_buffers; /* planet-block buffers (0 or < 2 means 2: the reference's double buffer) */
 } glx_params;
 
+/* One record of the host close-encounter queue (glx_pack_encounters,
+ * glx_exec_take_encounters). */
+typedef struct glx_encounter {
+    uint32_t id;
+    uint16_t flags;     /* 0x01, planet id in the high byte (0: the Sun) */
+    uint32_t step;      /* steps of the block completed when flagged */
+    double t_block;     /* time at the start of that block */
+    double r0[3], v0[3];  /* state at the start of the block (rollback copy) */
+    double r[3], v[3];    /* state at the flagging step */
+} glx_encounter;
+
 /* Create / destroy a context and all its device memory.
  * Replaces the allocations of Executor::init (src/executor.cu:151-175) and
  * WHCudaIntegrator's constructor (src/wh.cu:272-282). */
@@ -179,6 +190,13 @@ int glx_partition(glx_ctx* ctx, int mode, uint64_t* n_unflagged, uint64_t* n_enc
 /* Copy r, v, flags, deathtime_index and id of [0, n_total) into the rollback
  * copy, async.  Replaces the memcpy_dtd block of src/executor.cu:624-635. */
 int glx_snapshot_rollback(glx_ctx* ctx);
+
+/* Particles [begin, begin+n) as encounter records, synchronous: packed on
+ * the device (state at the block start from the rollback copy when one is
+ * kept, else the current state; current state; flags; id; deathtime index
+ * as `step`), one copy to the host.  The download half of resync2 for the
+ * encounter range (src/executor.cu:812-849). */
+int glx_pack_encounters(glx_ctx* ctx, uint64_t begin, uint64_t n, double t_block, glx_encounter* out);
 
 /* Download the rollback copy of [begin, begin+n) (r, v may be NULL). */
 int glx_download_rollback(glx_ctx* ctx, uint64_t begin, uint64_t n, double* r, double* v);
@@ -305,16 +323,6 @@ typedef struct glx_exec_params {
     int32_t form;               /* glx_form */
     int32_t device;
 } glx_exec_params;
-
-/* One record of the host close-encounter queue. */
-typedef struct glx_encounter {
-    uint32_t id;
-    uint16_t flags;     /* 0x01, planet id in the high byte (0: the Sun) */
-    uint32_t step;      /* steps of the block completed when flagged */
-    double t_block;     /* time at the start of that block */
-    double r0[3], v0[3];  /* state at the start of the block (rollback copy) */
-    double r[3], v[3];    /* state at the flagging step */
-} glx_encounter;
 
 /* Create the loop: planets (n_bodies GM values, heliocentric r, v), n
  * particles (heliocentric pr, pv, ids).  Uploads, computes the initial

@@ -37,15 +37,11 @@ struct ExecutorConfig {
     int device = 0;
 };
 
-// One particle handed to the host close-encounter queue.
-struct EncounterRecord {
-    uint32_t id;
-    uint16_t flags;    // 0x01 with the planet id in the high byte (0: the Sun)
-    uint32_t step;     // steps of the block completed when it was flagged
-    double t_block;    // time at the start of that block
-    double r0[3], v0[3];  // state at the start of the block (rollback copy)
-    double r[3], v[3];    // state at the flagging step
-};
+// One particle handed to the host close-encounter queue: id, flags (0x01,
+// planet id in the high byte), step (steps of its block completed when
+// flagged), t_block (time at the start of that block), r0/v0 (state at the
+// start of the block, the rollback copy) and r/v (state at the flagging step).
+using EncounterRecord = glx_encounter;
 
 // Host copy of every particle slot (r, v: n x 3).
 struct ParticleState {
