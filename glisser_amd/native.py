@@ -75,6 +75,13 @@ class BatchConfig(C.Structure):
 
 
 BATCH_MAX_BODIES = 9
+STATE_ASCII, STATE_BINARY, STATE_SPLIT = 0, 1, 2
+
+
+class StateC(C.Structure):
+    """glx_state: caller-owned arrays of a state file."""
+    _fields_ = [("n_planets", C.c_uint32), ("n_particles", C.c_uint64), ("pm", _dp), ("pr", _dp), ("pv", _dp),
+                ("pid", _u32p), ("r", _dp), ("v", _dp), ("id", _u32p), ("flags", _u16p), ("deathtime", _dp)]
 
 # numpy view of glx_encounter records (layout of EncounterRec below)
 _ENC_DTYPE = np.dtype({"names": ["id", "flags", "step", "t_block", "r0", "v0", "r", "v"],
@@ -160,6 +167,12 @@ SIGNATURES = {
     "glx_exec_download": (C.c_int, [C.c_void_p, _dp, _dp, _u16p, _u32p, _dp]),
     "glx_exec_planets": (C.c_int, [C.c_void_p, _dp]),
     "glx_exec_times": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "glx_config_normalize": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint64, _u64p]),
+    "glx_state_save": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(StateC)]),
+    "glx_state_sizes": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_uint64, _u32p, _u64p]),
+    "glx_state_load": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_uint64, C.c_int, C.c_double,
+                                 C.POINTER(StateC)]),
+    "glx_track_write": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.POINTER(StateC), C.c_int, C.c_int]),
     "glx_planets_create": (C.c_int, [C.c_uint32, _dp, _dp, _dp, _u32p, C.c_double, C.c_double, C.c_double,
                                      C.POINTER(C.c_void_p)]),
     "glx_planets_destroy": (C.c_int, [C.c_void_p]),
@@ -640,6 +653,61 @@ class Executor:
         k, q, p = C.c_double(), C.c_double(), C.c_double()
         self._check(self.lib.glx_exec_times(self.h, C.byref(k), C.byref(q), C.byref(p)))
         return k.value, q.value, p.value
+
+
+def _state_c(st: dict):
+    """glx_state over numpy arrays (returned with the arrays it points to)."""
+    n = len(st["id"])
+    a = {"pm": np.ascontiguousarray(st["pm"], np.float64), "pr": np.ascontiguousarray(st["pr"], np.float64),
+         "pv": np.ascontiguousarray(st["pv"], np.float64),
+         "pid": np.ascontiguousarray(st.get("pid", np.arange(len(st["pm"]))), np.uint32),
+         "r": np.ascontiguousarray(st["r"], np.float64), "v": np.ascontiguousarray(st["v"], np.float64),
+         "id": np.ascontiguousarray(st["id"], np.uint32),
+         "flags": np.ascontiguousarray(st.get("flags", np.zeros(n)), np.uint16),
+         "deathtime": np.ascontiguousarray(st.get("deathtime", np.zeros(n)), np.float64)}
+    c = StateC(len(a["pm"]), n, _ptr(a["pm"], _dp), _ptr(a["pr"], _dp), _ptr(a["pv"], _dp), _ptr(a["pid"], _u32p),
+               _ptr(a["r"], _dp), _ptr(a["v"], _dp), _ptr(a["id"], _u32p), _ptr(a["flags"], _u16p),
+               _ptr(a["deathtime"], _dp))
+    return c, a
+
+
+def config_normalize(text: str) -> str:
+    """Parse a configuration text and return its canonical form."""
+    lib = load()
+    buf = C.create_string_buffer(1 << 16)
+    n = C.c_uint64(0)
+    _check_host(lib.glx_config_normalize(text.encode(), buf, len(buf), C.byref(n)))
+    return buf.value.decode()
+
+
+def save_state(path, st: dict, fmt: int = STATE_ASCII, write_momenta: bool = False, particle_path=None) -> None:
+    c, _keep = _state_c(st)
+    _check_host(load().glx_state_save(str(path).encode(),
+                                      None if particle_path is None else str(particle_path).encode(), fmt,
+                                      int(write_momenta), C.byref(c)))
+
+
+def load_state(path, fmt: int = STATE_ASCII, particle_path=None, max_particles: int = 0xFFFFFFFF,
+               read_momenta: bool = False, big_g: float = 1.0) -> dict:
+    lib = load()
+    pp = None if particle_path is None else str(particle_path).encode()
+    npl, n = C.c_uint32(0), C.c_uint64(0)
+    _check_host(lib.glx_state_sizes(str(path).encode(), pp, fmt, max_particles, C.byref(npl), C.byref(n)))
+    npl, n = npl.value, n.value
+    st = {"pm": np.empty(npl), "pr": np.empty((npl, 3)), "pv": np.empty((npl, 3)), "pid": np.empty(npl, np.uint32),
+          "r": np.empty((n, 3)), "v": np.empty((n, 3)), "id": np.empty(n, np.uint32), "flags": np.empty(n, np.uint16),
+          "deathtime": np.empty(n)}
+    c, _keep = _state_c(st)
+    _check_host(lib.glx_state_load(str(path).encode(), pp, fmt, max_particles, int(read_momenta), big_g,
+                                   C.byref(c)))
+    return st
+
+
+def write_track(path, time: float, st: dict, to_elements: bool = True, barycentric: bool = True,
+                append: bool = False) -> None:
+    c, _keep = _state_c(st)
+    _check_host(load().glx_track_write(str(path).encode(), int(append), time, C.byref(c), int(to_elements),
+                                       int(barycentric)))
 
 
 def load_h0(m, r_log) -> np.ndarray:

@@ -348,6 +348,58 @@ int glx_exec_planets(glx_exec* e, double* r);
 int glx_exec_times(glx_exec* e, double* kernel_ms, double* partition_ms, double* planet_ms);
 
 /* ---------------------------------------------------------------------
+ * File formats of the reference (configuration, state, binary track):
+ * src/data.cpp:146-815, README.md.  The C++ API is in
+ * glisser_amd/csrc/host/formats.h; these are its C entry points.
+ * ------------------------------------------------------------------- */
+
+/* Parse a configuration text (the reference's "Key value" lines and
+ * consistency rules, src/data.cpp:195-333) and write its canonical form
+ * (every key, src/data.cpp:335-371) into out (capacity bytes incl. the NUL);
+ * *length receives the text length.  GLX_EINVAL with the reference's message
+ * on a bad line or an inconsistent configuration. */
+int glx_config_normalize(const char* text, char* out, uint64_t capacity, uint64_t* length);
+
+enum glx_state_format {
+    GLX_STATE_ASCII = 0,  /* one file, text: src/data.cpp:427-483,620-643 */
+    GLX_STATE_BINARY = 1, /* one file, binary: src/data.cpp:485-525,586-618 */
+    GLX_STATE_SPLIT = 2   /* planet file + particle file, text: src/data.cpp:374-425,645-668 */
+};
+
+/* Caller-owned arrays of a state: planets (n_planets GM values, r, v as
+ * n x 3, ids) and particles (r, v, ids, flags, deathtime). */
+typedef struct glx_state {
+    uint32_t n_planets;
+    uint64_t n_particles;
+    double* pm;
+    double* pr;
+    double* pv;
+    uint32_t* pid;     /* NULL on save: 0..n-1 */
+    double* r;
+    double* v;
+    uint32_t* id;
+    uint16_t* flags;   /* NULL on save: zeros */
+    double* deathtime; /* NULL on save: zeros */
+} glx_state;
+
+/* Write a state file (split: path = planets, particle_path = particles);
+ * write_momenta multiplies planet velocities by GM. */
+int glx_state_save(const char* path, const char* particle_path, int format, int write_momenta, const glx_state* s);
+/* Sizes of a state file (at most max_particles particles). */
+int glx_state_sizes(const char* path, const char* particle_path, int format, uint64_t max_particles,
+                    uint32_t* n_planets, uint64_t* n_particles);
+/* Read a state file into arrays sized by glx_state_sizes, then divide
+ * planet momenta by GM (read_momenta) and scale GM by big_g (load_data,
+ * src/data.cpp:527-584).  The binary deathtime is read as the double the
+ * writer stores (the reference reads a float there). */
+int glx_state_load(const char* path, const char* particle_path, int format, uint64_t max_particles, int read_momenta,
+                   double big_g, glx_state* s);
+/* Append (or start) a binary track file with one snapshot of the state's
+ * planets and particles, in the given particle order (save_binary_track,
+ * src/data.cpp:718-815). */
+int glx_track_write(const char* path, int append, double time, const glx_state* s, int to_elements, int barycentric);
+
+/* ---------------------------------------------------------------------
  * Host-side producer of the kernel's planet input and orbital conversions.
  * Built without FP contraction so it rounds like the reference CPU build.
  * ------------------------------------------------------------------- */

@@ -39,6 +39,15 @@ _SIGS = {
     "to_elements": (None, [C.c_double, _dp, _dp, _dp]),
     "from_elements": (None, [C.c_double, _dp, _dp, _dp]),
 }
+_REF_ONLY = {
+    "config_normalize": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint64, _u64p]),
+    "state_save": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_uint32, _dp, _dp, _dp, _u32p, C.c_uint64,
+                             _dp, _dp, _u32p, _u16p, _dp]),
+    "state_load": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_uint32, C.c_uint64, _u32p, _dp, _dp, _dp, _u32p,
+                             _u64p, _dp, _dp, _u32p, _u16p, _dp]),
+    "track_write": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_uint32, _dp, _dp, _dp, _u32p, C.c_uint64, _dp,
+                              _dp, _u32p, C.c_int, C.c_int]),
+}
 _ORACLE_ONLY = {
     "drift": (C.c_int, [C.c_double, C.c_uint32, _dp, _dp, _dp, C.c_int, _u64p]),
     "hermite_log": (C.c_int, [C.c_uint32, _dp, C.c_uint64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_uint32,
@@ -69,8 +78,7 @@ class Oracle:
         self.prefix = "ref_" if reference else "orc_"
         self.reference = reference
         sigs = dict(_SIGS)
-        if not reference:
-            sigs.update(_ORACLE_ONLY)
+        sigs.update(_REF_ONLY if reference else _ORACLE_ONLY)
         for name, (res, args) in sigs.items():
             fn = getattr(self.lib, self.prefix + name)
             fn.restype = res
@@ -147,6 +155,48 @@ class Oracle:
             raise OracleError(rc, self.err())
         return rl, h0
 
+    # -- file formats (reference build only) ---------------------------------
+    def config_normalize(self, text: str) -> str:
+        """The reference's read_configuration + write_configuration."""
+        buf = C.create_string_buffer(1 << 16)
+        n = C.c_uint64(0)
+        rc = self._f("config_normalize")(text.encode(), buf, len(buf), C.byref(n))
+        if rc != 0:
+            raise OracleError(rc, self.err())
+        return buf.value.decode()
+
+    def state_save(self, path, fmt: int, st: dict, write_momenta=False, particle_path=None) -> None:
+        a = _state_arrays(st)
+        self._f("state_save")(str(path).encode(), None if particle_path is None else str(particle_path).encode(),
+                              fmt, int(write_momenta), len(a["pm"]), _p(a["pm"], _dp), _p(a["pr"], _dp),
+                              _p(a["pv"], _dp), _p(a["pid"], _u32p), len(a["id"]), _p(a["r"], _dp),
+                              _p(a["v"], _dp), _p(a["id"], _u32p), _p(a["flags"], _u16p), _p(a["deathtime"], _dp))
+
+    def state_load(self, path, fmt: int, particle_path=None, cap_pl=64, cap=1 << 16) -> dict:
+        out = {"pm": np.empty(cap_pl), "pr": np.empty((cap_pl, 3)), "pv": np.empty((cap_pl, 3)),
+               "pid": np.empty(cap_pl, np.uint32), "r": np.empty((cap, 3)), "v": np.empty((cap, 3)),
+               "id": np.empty(cap, np.uint32), "flags": np.empty(cap, np.uint16), "deathtime": np.empty(cap)}
+        npl, n = C.c_uint32(0), C.c_uint64(0)
+        rc = self._f("state_load")(str(path).encode(), None if particle_path is None else str(particle_path).encode(),
+                                   fmt, cap_pl, cap, C.byref(npl), _p(out["pm"], _dp), _p(out["pr"], _dp),
+                                   _p(out["pv"], _dp), _p(out["pid"], _u32p), C.byref(n), _p(out["r"], _dp),
+                                   _p(out["v"], _dp), _p(out["id"], _u32p), _p(out["flags"], _u16p),
+                                   _p(out["deathtime"], _dp))
+        if rc != 0:
+            raise OracleError(rc, "state_load")
+        for k in ("pm", "pr", "pv", "pid"):
+            out[k] = out[k][:npl.value]
+        for k in ("r", "v", "id", "flags", "deathtime"):
+            out[k] = out[k][:n.value]
+        return out
+
+    def track_write(self, path, time: float, st: dict, to_elements=True, barycentric=True, append=False) -> None:
+        a = _state_arrays(st)
+        self._f("track_write")(str(path).encode(), int(append), time, len(a["pm"]), _p(a["pm"], _dp),
+                               _p(a["pr"], _dp), _p(a["pv"], _dp), _p(a["pid"], _u32p), len(a["id"]),
+                               _p(a["r"], _dp), _p(a["v"], _dp), _p(a["id"], _u32p), int(to_elements),
+                               int(barycentric))
+
     # -- planets ----------------------------------------------------------
     def planets(self, m, r, v, ids=None, cull_radius=0.5, rh_factor=0.0, outer=300.0):
         return OraclePlanets(self, m, r, v, ids, cull_radius, rh_factor, outer)
@@ -186,6 +236,16 @@ class Oracle:
         r, v = np.empty(3), np.empty(3)
         self._f("from_elements")(mu, _p(el, _dp), _p(r, _dp), _p(v, _dp))
         return r, v
+
+
+def _state_arrays(st: dict) -> dict:
+    n = len(st["id"])
+    return {"pm": np.ascontiguousarray(st["pm"], np.float64), "pr": np.ascontiguousarray(st["pr"], np.float64),
+            "pv": np.ascontiguousarray(st["pv"], np.float64), "pid": np.ascontiguousarray(st["pid"], np.uint32),
+            "r": np.ascontiguousarray(st["r"], np.float64), "v": np.ascontiguousarray(st["v"], np.float64),
+            "id": np.ascontiguousarray(st["id"], np.uint32),
+            "flags": np.ascontiguousarray(st.get("flags", np.zeros(n)), np.uint16),
+            "deathtime": np.ascontiguousarray(st.get("deathtime", np.zeros(n)), np.float64)}
 
 
 class OraclePlanets:

@@ -18,8 +18,27 @@
 #include <cstdint>
 #include <cstring>
 #include <cstdio>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
+
+// Defined in the reference's src/data.cpp (external linkage) without a
+// declaration in its headers.
+namespace sr {
+namespace data {
+bool load_data_nohybrid(HostPlanetPhaseSpace& pl, HostParticlePhaseSpace& pa, const Configuration& config,
+                        std::istream& plin, std::istream& icsin);
+bool load_data_hybrid(HostPlanetPhaseSpace& pl, HostParticlePhaseSpace& pa, const Configuration& config,
+                      std::istream& in);
+void save_data_hybrid_binary(const HostPlanetSnapshot& pl, const HostParticlePhaseSpace& pa,
+                             const Configuration& config, std::ostream& out);
+void save_data_hybrid(const HostPlanetSnapshot& pl, const HostParticlePhaseSpace& pa, const Configuration& config,
+                      std::ostream& out);
+void save_data_nohybrid(const HostPlanetSnapshot& pl, const HostParticlePhaseSpace& pa, const Configuration& config,
+                        std::ostream& plout, std::ostream& icsout);
+}  // namespace data
+}  // namespace sr
 
 using namespace sr::data;
 using namespace sr::wh;
@@ -238,6 +257,126 @@ void ref_from_elements(double mu, const double* el6, double* r, double* v) {
     sr::convert::from_elements(mu, el6[0], el6[1], el6[2], el6[3], el6[4], el6[5], &R, &V);
     st(r, 0, R);
     st(v, 0, V);
+}
+
+// ---- file formats (src/data.cpp:146-815) ----------------------------------
+
+// read_configuration + write_configuration of a configuration text; returns
+// 0, or 1 with the exception text in ref_last_error.
+int ref_config_normalize(const char* text, char* out, uint64_t cap, uint64_t* len) {
+    try {
+        Configuration c;
+        std::istringstream in(text);
+        read_configuration(in, &c);
+        std::ostringstream o;
+        write_configuration(o, c);
+        const std::string s = o.str();
+        *len = s.size();
+        if (cap < s.size() + 1) return 2;
+        std::memcpy(out, s.c_str(), s.size() + 1);
+        return 0;
+    } catch (const std::exception& e) {
+        std::snprintf(g_err, sizeof g_err, "%s", e.what());
+        return 1;
+    }
+}
+
+namespace {
+
+void fill_planets(HostPlanetSnapshot& pl, uint32_t np, const double* pm, const double* pr, const double* pv,
+                  const uint32_t* pid) {
+    pl = HostPlanetSnapshot(np);
+    for (uint32_t i = 0; i < np; i++) {
+        pl.m[i] = pm[i];
+        pl.r[i] = ld(pr, i);
+        pl.v[i] = ld(pv, i);
+        pl.id[i] = pid[i];
+    }
+}
+
+}  // namespace
+
+// save_data_hybrid / save_data_hybrid_binary / save_data_nohybrid
+// (format 0 / 1 / 2; format 2 writes path and ppath).
+int ref_state_save(const char* path, const char* ppath, int format, int write_momenta, uint32_t np,
+                   const double* pm, const double* pr, const double* pv, const uint32_t* pid, uint64_t n,
+                   const double* r, const double* v, const uint32_t* id, const uint16_t* flags,
+                   const double* deathtime) {
+    HostPlanetSnapshot pl;
+    fill_planets(pl, np, pm, pr, pv, pid);
+    HostParticlePhaseSpace pa(n);
+    for (uint64_t i = 0; i < n; i++) {
+        pa.r()[i] = ld(r, i);
+        pa.v()[i] = ld(v, i);
+        pa.id()[i] = id[i];
+        pa.deathflags()[i] = flags[i];
+        pa.deathtime_map()[id[i]] = deathtime[i];
+    }
+    Configuration cfg;
+    cfg.writemomenta = write_momenta != 0;
+    if (format == 2) {
+        std::ofstream plo(path), pao(ppath);
+        save_data_nohybrid(pl, pa, cfg, plo, pao);
+    } else if (format == 1) {
+        std::ofstream o(path, std::ios_base::binary);
+        save_data_hybrid_binary(pl, pa, cfg, o);
+    } else {
+        std::ofstream o(path);
+        save_data_hybrid(pl, pa, cfg, o);
+    }
+    return 0;
+}
+
+// load_data_hybrid / load_data_nohybrid (format 0 / 2) into caller arrays
+// sized for at most cap_pl planets and cap particles; *np, *n receive the
+// counts (deathtime by particle order).
+int ref_state_load(const char* path, const char* ppath, int format, uint32_t cap_pl, uint64_t cap, uint32_t* np,
+                   double* pm, double* pr, double* pv, uint32_t* pid, uint64_t* n, double* r, double* v,
+                   uint32_t* id, uint16_t* flags, double* deathtime) {
+    HostPlanetPhaseSpace pl;
+    HostParticlePhaseSpace pa;
+    Configuration cfg = Configuration::create_dummy();
+    if (format == 2) {
+        std::ifstream pli(path), pai(ppath);
+        load_data_nohybrid(pl, pa, cfg, pli, pai);
+    } else {
+        std::ifstream in(path);
+        load_data_hybrid(pl, pa, cfg, in);
+    }
+    if (pl.n() > cap_pl || pa.n() > cap) return 2;
+    *np = (uint32_t)pl.n();
+    for (size_t i = 0; i < pl.n(); i++) {
+        pm[i] = pl.m()[i];
+        st(pr, i, pl.r()[i]);
+        st(pv, i, pl.v()[i]);
+        pid[i] = pl.id()[i];
+    }
+    *n = pa.n();
+    for (size_t i = 0; i < pa.n(); i++) {
+        st(r, i, pa.r()[i]);
+        st(v, i, pa.v()[i]);
+        id[i] = pa.id()[i];
+        flags[i] = pa.deathflags()[i];
+        deathtime[i] = format == 2 ? pa.deathtime_map()[(uint32_t)i] : pa.deathtime_map()[pa.id()[i]];
+    }
+    return 0;
+}
+
+// save_binary_track of one snapshot (particles in the given order), appended.
+int ref_track_write(const char* path, int append, double time, uint32_t np, const double* pm, const double* pr,
+                    const double* pv, const uint32_t* pid, uint64_t n, const double* r, const double* v,
+                    const uint32_t* id, int to_elements, int barycentric) {
+    HostPlanetSnapshot pl;
+    fill_planets(pl, np, pm, pr, pv, pid);
+    HostParticleSnapshot pa(n);
+    for (uint64_t i = 0; i < n; i++) {
+        pa.r[i] = ld(r, i);
+        pa.v[i] = ld(v, i);
+        pa.id[i] = id[i];
+    }
+    std::ofstream o(path, std::ios_base::binary | (append ? std::ios_base::app : std::ios_base::trunc));
+    save_binary_track(o, pl, pa, time, to_elements != 0, barycentric != 0);
+    return 0;
 }
 
 }  // extern "C"

@@ -6,9 +6,11 @@ into oracle/_ref/libglisser_ref.so in the container that holds the
 reference tree.  Inputs are seeded synthetic systems.  Run from the repo
 root:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [part ...]
 
-The fixtures are small .npz files (inputs and expected outputs only).
+The fixtures are small .npz files (inputs and expected outputs only) and,
+under formats/, state, configuration and track files written by the
+reference's own writers.
 """
 from __future__ import annotations
 
@@ -109,13 +111,26 @@ def elements(ref):
     np.savez_compressed(OUT / "elements.npz", mu=mu, el=el, r=r, v=v, el_back=back)
 
 
+def formats(ref):
+    """State files, a configuration and a track snapshot written by the
+    reference's own writers (src/data.cpp) from tests/test_formats.py's
+    sample state."""
+    from tests.test_formats import CONFIG, sample_state
+    d = OUT / "formats"
+    d.mkdir(exist_ok=True)
+    st = sample_state()
+    ref.state_save(d / "state.txt", 0, st)
+    ref.state_save(d / "state.bin", 1, st)
+    ref.state_save(d / "pl.txt", 2, st, particle_path=d / "ics.txt")
+    ref.track_write(d / "track.bin", 123.25, st, True, True)
+    (d / "config.out").write_text(ref.config_normalize(CONFIG))
+
+
 if __name__ == "__main__":
     ref = Oracle(reference=True)
-    planets(ref)
-    particle_block(ref)
-    long_run(ref)
-    encounters(ref)
-    partition(ref)
-    elements(ref)
-    for f in sorted(OUT.glob("*.npz")):
+    parts = sys.argv[1:] or ["planets", "particle_block", "long_run", "encounters", "partition", "elements",
+                             "formats"]
+    for name in parts:
+        globals()[name](ref)
+    for f in sorted(OUT.glob("*.npz")) + sorted((OUT / "formats").glob("*")):
         print(f.name, f.stat().st_size)
