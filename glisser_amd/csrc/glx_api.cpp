@@ -112,6 +112,10 @@ struct glx_ctx {
     uint64_t n_alive = 0, n_total = 0;
     History hist;
     double hist_t0 = 0, hist_t1 = 0;  // first and last history times (host copy)
+    glx::TrackRecordDev* trk_dev = nullptr;  // track records (grown on demand)
+    uint64_t trk_cap = 0;
+    uint32_t* rank_dev = nullptr;  // id ranks + scan tiles (grown on demand)
+    uint64_t rank_cap = 0;
     glx_encounter* enc_dev = nullptr;  // encounter-record staging (grown on demand)
     glx_encounter* enc_pin = nullptr;
     uint64_t enc_cap = 0;
@@ -158,6 +162,8 @@ void release(glx_ctx* c) {
         if (p) (void)hipFree(p);
     if (c->enc_dev) (void)hipFree(c->enc_dev);
     if (c->enc_pin) (void)hipHostFree(c->enc_pin);
+    if (c->trk_dev) (void)hipFree(c->trk_dev);
+    if (c->rank_dev) (void)hipFree(c->rank_dev);
     for (auto& b : c->pb) {
         if (b.pin_log) (void)hipHostFree(b.pin_log);
         if (b.uploaded) (void)hipEventDestroy(b.uploaded);
@@ -832,6 +838,74 @@ int glx_sum_steps(glx_ctx* c, uint64_t n, uint64_t* total) {
     unsigned long long v;
     std::memcpy(&v, c->pin_small + 4, sizeof v);
     *total = v;
+    return GLX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- tracks ----
+
+#include "host/formats.h"
+
+namespace {
+
+template <typename T>
+int grow(T** dev, uint64_t* cap, uint64_t n) {
+    if (n <= *cap) return GLX_OK;
+    if (*dev) (void)hipFree(*dev);
+    *dev = nullptr;
+    *cap = 0;
+    if (hipMalloc((void**)dev, std::max<uint64_t>(n, 4096) * sizeof(T)) != hipSuccess)
+        return fail(GLX_ENOMEM, "hipMalloc of %llu track entries failed", (unsigned long long)n);
+    *cap = std::max<uint64_t>(n, 4096);
+    return GLX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glx_track_pack(glx_ctx* c, uint32_t n_planets, const double* pm, const double* pr, const double* pv,
+                   int to_elements, int barycentric, void* records, uint64_t capacity, uint64_t* n_out) {
+    if (!c || !n_out) return fail(GLX_EINVAL, "null argument");
+    const uint64_t n = c->n_alive;
+    *n_out = n;
+    if (n == 0) return GLX_OK;
+    if (!records || capacity < n) return fail(GLX_EINVAL, "record buffer holds %llu, %llu needed",
+                                              (unsigned long long)capacity, (unsigned long long)n);
+    if (n_planets < 1 || !pm || !pr || !pv) return fail(GLX_EINVAL, "the planets (Sun first) are required");
+    glx::host::State pl;
+    pl.pm.assign(pm, pm + n_planets);
+    pl.pr.assign(pr, pr + 3 * (size_t)n_planets);
+    pl.pv.assign(pv, pv + 3 * (size_t)n_planets);
+    glx::TrackCenter center{};
+    glx::host::track_center(pl, to_elements && barycentric, &center.mass, center.r, center.v);
+    GLX_HIP(hipSetDevice(c->device));
+    if (int rc = grow(&c->trk_dev, &c->trk_cap, n)) return rc;
+    // ids ranked on the device when they are dense (max id < 4 x capacity)
+    GLX_HIP(glx::launch_max_id(c->cur.id, (uint32_t)n, c->totals, c->main_s));
+    GLX_HIP(hipMemcpyAsync(c->pin_small, c->totals, sizeof(uint32_t), hipMemcpyDeviceToHost, c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    const uint32_t max_id = c->pin_small[0];
+    const bool dense = (uint64_t)max_id < 4 * c->cap;
+    uint32_t* rank = nullptr;
+    if (dense) {
+        const uint32_t n_ids = max_id + 1;
+        if (int rc = grow(&c->rank_dev, &c->rank_cap, n_ids + glx::rank_scratch_tiles(n_ids))) return rc;
+        rank = c->rank_dev;
+        GLX_HIP(glx::launch_rank_ids(c->cur.id, (uint32_t)n, n_ids, rank, rank + n_ids, c->main_s));
+        // unique ids: the largest has rank n - 1
+        GLX_HIP(hipMemcpyAsync(c->pin_small, rank + max_id, sizeof(uint32_t), hipMemcpyDeviceToHost, c->main_s));
+        GLX_HIP(hipStreamSynchronize(c->main_s));
+        if (c->pin_small[0] != n - 1) return fail(GLX_ESTATE, "particle ids are not unique");
+    }
+    GLX_HIP(glx::launch_pack_track(c->cur, (uint32_t)n, rank, center, to_elements, c->trk_dev, c->main_s));
+    GLX_HIP(hipMemcpyAsync(records, c->trk_dev, n * sizeof(glx::TrackRecordDev), hipMemcpyDeviceToHost, c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    if (!dense) {
+        auto* r = static_cast<glx::TrackRecordDev*>(records);
+        std::sort(r, r + n, [](const glx::TrackRecordDev& a, const glx::TrackRecordDev& b) { return a.id < b.id; });
+    }
     return GLX_OK;
 }
 

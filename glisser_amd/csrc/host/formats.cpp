@@ -369,15 +369,11 @@ void apply_input_units(State& s, bool read_momenta, double big_g) {
     }
 }
 
-void write_track_snapshot(std::ostream& out, double time, const State& s, bool to_elements, bool bary) {
-    put_track_header_and_planets(out, time, s, to_elements, bary);
-    const size_t n = s.n_particles();
-    put<uint64_t>(out, n);
-    // particle centre: the Sun, or the barycentre of all bodies with their total mass
+void track_center(const State& s, bool bary, double* mass, double r[3], double v[3]) {
     double cm = s.n_planets() ? s.pm[0] : 0.0;
     Vec3 cr = s.n_planets() ? Vec3::load(s.pr.data(), 0) : Vec3();
     Vec3 cv = s.n_planets() ? Vec3::load(s.pv.data(), 0) : Vec3();
-    if (to_elements && bary) {
+    if (bary) {
         Vec3 rs, vs;
         double ms = 0;
         for (size_t j = 0; j < s.n_planets(); j++) {
@@ -389,6 +385,18 @@ void write_track_snapshot(std::ostream& out, double time, const State& s, bool t
         cv = vs / ms;
         cm = ms;
     }
+    *mass = cm;
+    cr.store(r, 0);
+    cv.store(v, 0);
+}
+
+void write_track_snapshot(std::ostream& out, double time, const State& s, bool to_elements, bool bary) {
+    put_track_header_and_planets(out, time, s, to_elements, bary);
+    const size_t n = s.n_particles();
+    put<uint64_t>(out, n);
+    double cm = 0, crv[3], cvv[3];
+    track_center(s, to_elements && bary, &cm, crv, cvv);
+    const Vec3 cr(crv[0], crv[1], crv[2]), cv(cvv[0], cvv[1], cvv[2]);
     for (size_t i = 0; i < n; i++) {
         TrackRecord rec;
         rec.id = s.id[i];

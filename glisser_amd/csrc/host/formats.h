@@ -69,6 +69,10 @@ void apply_input_units(State& s, bool read_momenta, double big_g);
 // Cartesian state.  Planet 0 (the Sun) is the centre and is not written.
 void write_track_snapshot(std::ostream& out, double time, const State& s, bool to_elements, bool barycentric);
 
+// Centre of the particle elements of a track snapshot: the Sun (GM m[0]),
+// or with barycentric the barycentre of all bodies with their total GM.
+void track_center(const State& planets, bool barycentric, double* mass, double r[3], double v[3]);
+
 // 28-byte particle record of a track snapshot.
 #pragma pack(push, 1)
 struct TrackRecord {

@@ -156,3 +156,21 @@ int glx_track_write(const char* path, int append, double time, const glx_state* 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int glx_track_write_records(const char* path, int append, double time, const glx_state* s, int to_elements,
+                            int barycentric, const void* records, uint64_t n) {
+    if (!s || !path || (n && !records)) return glx::set_error(GLX_EINVAL, "null argument");
+    if (s->n_planets && (!s->pm || !s->pr || !s->pv)) return glx::set_error(GLX_EINVAL, "null planet array");
+    return guarded([&]() -> int {
+        glx_state planets = *s;
+        planets.n_particles = 0;
+        std::ofstream out(path, std::ios::binary | (append ? std::ios::app : std::ios::trunc));
+        glx::host::write_track_records(out, time, from_c(&planets), to_elements != 0, barycentric != 0,
+                                       static_cast<const glx::host::TrackRecord*>(records), n);
+        return out ? GLX_OK : glx::set_error(GLX_EINVAL, std::string("writing ") + path + " failed");
+    });
+}
+
+}  // extern "C"

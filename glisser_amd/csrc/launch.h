@@ -64,6 +64,32 @@ struct HistoryView {
 hipError_t launch_hermite_rows(const HistoryView& H, const double* times, double t_start, double dt, uint32_t n_steps,
                                double* out, hipStream_t stream);
 
+// Track snapshots on the device: the centre the elements refer to, and the
+// 28-byte record (u32 id + a, e, i, Omega, omega, f or x, y, z, vx, vy, vz as
+// float32).
+struct TrackCenter {
+    double mass;
+    double r[3];
+    double v[3];
+};
+struct TrackRecordDev {
+    uint32_t id;
+    float x[6];
+};
+static_assert(sizeof(TrackRecordDev) == 28, "track record is 28 bytes");
+
+// max of id[0, n) into *out (device).
+hipError_t launch_max_id(const uint32_t* id, uint32_t n, uint32_t* out, hipStream_t stream);
+// Tiles of the rank scan for ids [0, n_ids): size of the tile_sums scratch.
+uint32_t rank_scratch_tiles(uint32_t n_ids);
+// present[0, n_ids) becomes rank[id] = number of ids in id[0, n) below id
+// (ids unique and < n_ids).
+hipError_t launch_rank_ids(const uint32_t* id, uint32_t n, uint32_t n_ids, uint32_t* present, uint32_t* tile_sums,
+                           hipStream_t stream);
+// Records of particles [0, n) written at out[rank[id]] (rank null: slot order).
+hipError_t launch_pack_track(ParticleSet cur, uint32_t n, const uint32_t* rank, const TrackCenter& c, int to_elements,
+                             TrackRecordDev* out, hipStream_t stream);
+
 // AoS (x,y,z) <-> SoA transposes for host copies staged through device memory.
 hipError_t launch_aos_to_soa(const double* aos, double* x, double* y, double* z, uint32_t n, hipStream_t stream);
 hipError_t launch_soa_to_aos(const double* x, const double* y, const double* z, double* aos, uint32_t n,

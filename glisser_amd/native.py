@@ -173,6 +173,10 @@ SIGNATURES = {
     "glx_state_load": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_uint64, C.c_int, C.c_double,
                                  C.POINTER(StateC)]),
     "glx_track_write": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.POINTER(StateC), C.c_int, C.c_int]),
+    "glx_track_pack": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _dp, _dp, C.c_int, C.c_int, C.c_void_p, C.c_uint64,
+                                 _u64p]),
+    "glx_track_write_records": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.POINTER(StateC), C.c_int, C.c_int,
+                                          C.c_void_p, C.c_uint64]),
     "glx_planets_create": (C.c_int, [C.c_uint32, _dp, _dp, _dp, _u32p, C.c_double, C.c_double, C.c_double,
                                      C.POINTER(C.c_void_p)]),
     "glx_planets_destroy": (C.c_int, [C.c_void_p]),
@@ -362,6 +366,20 @@ class Context:
         r, v = np.empty((n, 3)), np.empty((n, 3))
         self._check(self.lib.glx_download_rollback(self.h, begin, n, _ptr(r, _dp), _ptr(v, _dp)))
         return r, v
+
+    def track_records(self, m, r, v, to_elements: bool = True, barycentric: bool = True) -> np.ndarray:
+        """Track records (structured: id, x[6] float32) of the alive particles,
+        ordered by id, formed on the device from planets m, r, v (Sun first)."""
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        r = _vec3(r, len(m))
+        v = _vec3(v, len(m))
+        n_alive, _ = self.counts()
+        out = np.empty(max(n_alive, 1), dtype=TRACK_DTYPE)
+        got = C.c_uint64(0)
+        self._check(self.lib.glx_track_pack(self.h, len(m), _ptr(m, _dp), _ptr(r, _dp), _ptr(v, _dp),
+                                            int(to_elements), int(barycentric), out.ctypes.data, len(out),
+                                            C.byref(got)))
+        return out[:got.value]
 
     def sync(self) -> None:
         self._check(self.lib.glx_sync(self.h))
@@ -701,6 +719,19 @@ def load_state(path, fmt: int = STATE_ASCII, particle_path=None, max_particles: 
     _check_host(lib.glx_state_load(str(path).encode(), pp, fmt, max_particles, int(read_momenta), big_g,
                                    C.byref(c)))
     return st
+
+
+TRACK_DTYPE = np.dtype([("id", np.uint32), ("x", np.float32, 6)])  # 28-byte track record
+
+
+def write_track_records(path, time: float, planets: dict, records: np.ndarray, to_elements: bool = True,
+                        barycentric: bool = True, append: bool = False) -> None:
+    """Track snapshot from the planets of `planets` and device-formed records."""
+    st = dict(planets, r=np.zeros((0, 3)), v=np.zeros((0, 3)), id=np.zeros(0, np.uint32))
+    c, _keep = _state_c(st)
+    rec = np.ascontiguousarray(records, dtype=TRACK_DTYPE)
+    _check_host(load().glx_track_write_records(str(path).encode(), int(append), time, C.byref(c), int(to_elements),
+                                               int(barycentric), rec.ctypes.data, len(rec)))
 
 
 def write_track(path, time: float, st: dict, to_elements: bool = True, barycentric: bool = True,

@@ -399,6 +399,20 @@ int glx_state_load(const char* path, const char* particle_path, int format, uint
  * src/data.cpp:718-815). */
 int glx_track_write(const char* path, int append, double time, const glx_state* s, int to_elements, int barycentric);
 
+/* Track records of the context's alive particles [0, n_alive) formed on the
+ * device, synchronous: elements about the Sun or the barycentre of the
+ * planets given (pm, pr, pv: n_planets bodies, Sun first), or the Cartesian
+ * state, as 28-byte records (u32 id + 6 float32) ordered by id -- the
+ * sort_by_id + save_binary_track particle loop of the reference
+ * (targets/main.cpp:148-156, src/data.cpp:770-812).  records holds capacity
+ * records; *n_out = n_alive.  GLX_ESTATE if particle ids repeat. */
+int glx_track_pack(glx_ctx* ctx, uint32_t n_planets, const double* pm, const double* pr, const double* pv,
+                   int to_elements, int barycentric, void* records, uint64_t capacity, uint64_t* n_out);
+/* Append (or start) a track file with one snapshot: the planets of s (its
+ * particles are not used) and n records from glx_track_pack. */
+int glx_track_write_records(const char* path, int append, double time, const glx_state* s, int to_elements,
+                            int barycentric, const void* records, uint64_t n);
+
 /* ---------------------------------------------------------------------
  * Host-side producer of the kernel's planet input and orbital conversions.
  * Built without FP contraction so it rounds like the reference CPU build.

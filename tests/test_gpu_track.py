@@ -1,0 +1,96 @@
+"""Track records formed on the device (glx_track_pack) against the host
+track writer, which writes the reference's bytes (tests/test_formats.py):
+ids in id order, Cartesian records bit for bit, element records within one
+float32 ulp (the device's acos/atan2 may round the last double bit
+differently from glibc)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def native():
+    from glisser_amd import native as nat
+    nat.load()
+    return nat
+
+
+def _swarm(n, seed):
+    from glisser_amd import systems
+    return systems.outer_solar_system(n, seed=seed, a_range=(8.0, 90.0), e_hi=0.6, i_hi_deg=60.0)
+
+
+def _ctx_with(native, s, ids, flags):
+    n = s.n
+    ctx = native.Context(n, s.n_bodies, 16)
+    ctx.upload(s.pr, s.pv, flags, ids, a=np.zeros((n, 3)), dti=np.zeros(n, np.uint32))
+    ctx.set_counts(n, n)
+    ctx.partition()  # alive = unflagged, in slot order
+    return ctx
+
+
+def _host_records(native, tmp_path, s, ids, flags, to_elements, bary):
+    alive = flags == 0
+    order = np.argsort(ids[alive])
+    st = {"pm": s.m, "pr": s.r, "pv": s.v, "pid": np.arange(s.n_bodies), "r": s.pr[alive][order],
+          "v": s.pv[alive][order], "id": ids[alive][order]}
+    path = tmp_path / "host.trk"
+    native.write_track(path, 10.0, st, to_elements, bary)
+    data = path.read_bytes()
+    head = 8 + 8 + (s.n_bodies - 1) * 28 + 8
+    return np.frombuffer(data[head:], dtype=native.TRACK_DTYPE), st
+
+
+def _ulps(a, b):
+    ai = a.view(np.int32).astype(np.int64)
+    bi = b.view(np.int32).astype(np.int64)
+    return np.abs(ai - bi)
+
+
+@pytest.mark.parametrize("to_elements,bary", [(True, True), (True, False), (False, False)])
+def test_device_records_match_host_writer(native, tmp_path, to_elements, bary):
+    s = _swarm(5000, 91)
+    rng = np.random.default_rng(91)
+    ids = (rng.permutation(s.n) + 3).astype(np.uint32)
+    flags = np.where(rng.random(s.n) < 0.2, 0x80, 0).astype(np.uint16)
+    ctx = _ctx_with(native, s, ids, flags)
+    try:
+        dev = ctx.track_records(s.m, s.r, s.v, to_elements, bary)
+    finally:
+        ctx.close()
+    host, st = _host_records(native, tmp_path, s, ids, flags, to_elements, bary)
+    np.testing.assert_array_equal(dev["id"], host["id"])
+    assert np.all(np.diff(dev["id"].astype(np.int64)) > 0)
+    u = _ulps(dev["x"], host["x"])
+    if not to_elements:
+        assert u.max() == 0
+    else:
+        assert u.max() <= 1 and np.mean(u == 0) > 0.999, (u.max(), np.mean(u == 0))
+    # a whole snapshot written from the device records has the host file's layout
+    native.write_track_records(tmp_path / "dev.trk", 10.0, st, dev, to_elements, bary)
+    a, b = (tmp_path / "dev.trk").read_bytes(), (tmp_path / "host.trk").read_bytes()
+    assert len(a) == len(b) and a[:8 + 8 + (s.n_bodies - 1) * 28 + 8] == b[:8 + 8 + (s.n_bodies - 1) * 28 + 8]
+
+
+def test_sparse_ids_and_errors(native):
+    s = _swarm(600, 92)
+    rng = np.random.default_rng(92)
+    ids = rng.choice(10**9, s.n, replace=False).astype(np.uint32)  # far above 4 x capacity: host ordering
+    flags = np.zeros(s.n, np.uint16)
+    ctx = _ctx_with(native, s, ids, flags)
+    try:
+        rec = ctx.track_records(s.m, s.r, s.v, False, False)
+        np.testing.assert_array_equal(rec["id"], np.sort(ids))
+        order = np.argsort(ids)
+        np.testing.assert_array_equal(rec["x"][:, :3], s.pr[order].astype(np.float32))
+    finally:
+        ctx.close()
+    dup = np.arange(s.n, dtype=np.uint32)
+    dup[5] = dup[6]
+    ctx = _ctx_with(native, s, dup, flags)
+    try:
+        with pytest.raises(native.GlxError, match="not unique"):
+            ctx.track_records(s.m, s.r, s.v)
+    finally:
+        ctx.close()
