@@ -103,7 +103,7 @@ def make_inputs(n_particles: int, n_blocks: int, seed: int):
     return s, hist, rh
 
 
-def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_simd: int):
+def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_simd: int, parts: int = 1):
     from glisser_amd import native
 
     s, hist, rh = inputs
@@ -120,7 +120,7 @@ def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_s
         if warmup:
             for k in range(warmup):
                 ctx.interp_block(k, k * TBSIZE * DT, DT, TBSIZE, rh)
-            ctx.run_blocks(warmup, 0, DT, OUTER, form, shards, waves_per_simd)
+            ctx.run_blocks(warmup, 0, DT, OUTER, form, shards, waves_per_simd, parts)
             ctx.kill_encounters()
             ctx.partition()
         ctx.sync()
@@ -128,7 +128,7 @@ def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_s
         t0 = time.perf_counter()
         for k in range(warmup, warmup + steps):
             ctx.interp_block(k, k * TBSIZE * DT, DT, TBSIZE, rh)
-        ctx.run_blocks(steps, warmup, DT, OUTER, form, shards, waves_per_simd)
+        ctx.run_blocks(steps, warmup, DT, OUTER, form, shards, waves_per_simd, parts)
         ctx.kill_encounters()
         psteps = ctx.sum_steps(n_alive)
         ctx.partition()
@@ -346,6 +346,8 @@ def main(argv=None):
                     help="swarm size (the benchmark workload is 200,000)")
     ap.add_argument("--shards", type=int, default=4, help="particle slices, one stream each")
     ap.add_argument("--waves-per-simd", type=int, default=3, help="resident-wave cap (0: none)")
+    ap.add_argument("--launches-per-block", type=int, default=4,
+                    help="launches each 2048-step block is split into (same results, shorter chain end)")
     ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--oracle-sample", type=int, default=256,
@@ -377,7 +379,7 @@ def main(argv=None):
     workgroups = (args.particles + 255) // 256
     per_form = {}
     for form in forms:
-        r = run_gpu(inputs, form, args.steps, args.warmup, args.shards, args.waves_per_simd)
+        r = run_gpu(inputs, form, args.steps, args.warmup, args.shards, args.waves_per_simd, args.launches_per_block)
         finals[form] = r.pop("final")
         block_s = r["kernel_ms"] / 1e3 / args.steps
         per_block = r["particle_steps"] / args.steps
@@ -418,6 +420,7 @@ def main(argv=None):
             "blocks_per_launch": args.steps,
             "shards": args.shards,
             "waves_per_simd": args.waves_per_simd,
+            "launches_per_block": args.launches_per_block,
             "planet_log": f"device cubic-Hermite interpolation of a {HIST_SPACING:g}-d history",
             "seed": args.seed,
             "parallelism": "single-gpu",

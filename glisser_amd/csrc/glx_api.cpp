@@ -469,7 +469,7 @@ int glx_run_block(glx_ctx* c, int buf, double dt, double outer_radius, int form)
 }
 
 int glx_run_blocks(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
-                   uint32_t n_shards, uint32_t waves_per_simd) {
+                   uint32_t n_shards, uint32_t waves_per_simd, uint32_t launches_per_block) {
     if (!c) return fail(GLX_EINVAL, "null context");
     if (int rc = check_form(form)) return rc;
     if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
@@ -479,6 +479,8 @@ int glx_run_blocks(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, doub
     if (n_shards < 1 || n_shards > kMaxShards) return fail(GLX_EINVAL, "n_shards %u outside [1, %d]", n_shards, kMaxShards);
     if (waves_per_simd != 0 && (waves_per_simd < 3 || waves_per_simd > 8))
         return fail(GLX_EINVAL, "waves_per_simd %u: use 0 (no cap) or 3..8", waves_per_simd);
+    if (launches_per_block == 0) launches_per_block = 1;
+    if (launches_per_block > 64) return fail(GLX_EINVAL, "launches_per_block %u above 64", launches_per_block);
     for (uint32_t k = 0; k < n_blocks; k++) {
         const int buf = (first_buf + (int)k) % nbuf;
         if (int rc = check_buf(c, buf)) return rc;
@@ -499,14 +501,28 @@ int glx_run_blocks(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, doub
     uint32_t offset = 0;
     for (uint32_t k = 0; k < n_blocks; k++) {
         PlanetBuf& b = c->pb[(first_buf + (int)k) % nbuf];
-        BlockArgs A = block_args(c, b, dt, outer_radius, budget);
-        A.step_offset = offset;
-        for (uint32_t s = 0; s < n_shards; s++) {
-            A.first = s * per;
-            A.n = std::min<uint32_t>(n, (s + 1) * per);
-            if (A.first >= A.n) continue;
-            GLX_HIP(hipStreamWaitEvent(c->shard_s[s], b.uploaded, 0));
-            GLX_HIP(glx::launch_block(A, form, lds_request, c->shard_s[s]));
+        const BlockArgs whole = block_args(c, b, dt, outer_radius, budget);
+        const size_t row = (size_t)b.n_bodies * 3;
+        // The block's rows run as launches_per_block consecutive launches per
+        // slice (the same steps, so the same results): the launch that ends
+        // the chain is that much shorter, and so is the time the last
+        // partly filled round of workgroups leaves the other SIMDs idle.
+        const uint32_t parts = std::min(launches_per_block, b.n_steps);
+        for (uint32_t j = 0; j < parts; j++) {
+            const uint32_t r0 = (uint32_t)((uint64_t)b.n_steps * j / parts);
+            const uint32_t r1 = (uint32_t)((uint64_t)b.n_steps * (j + 1) / parts);
+            BlockArgs A = whole;
+            A.log = b.dev_log + r0 * row;
+            A.n_steps = r1 - r0;
+            A.chunk = std::min(A.chunk, A.n_steps);
+            A.step_offset = offset + r0;
+            for (uint32_t s = 0; s < n_shards; s++) {
+                A.first = s * per;
+                A.n = std::min<uint32_t>(n, (s + 1) * per);
+                if (A.first >= A.n) continue;
+                if (j == 0) GLX_HIP(hipStreamWaitEvent(c->shard_s[s], b.uploaded, 0));
+                GLX_HIP(glx::launch_block(A, form, lds_request, c->shard_s[s]));
+            }
         }
         offset += b.n_steps;
     }

@@ -134,7 +134,7 @@ SIGNATURES = {
     "glx_download_planet_block": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, _dp, _dp]),
     "glx_run_block": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
     "glx_run_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint32,
-                                 C.c_uint32]),
+                                 C.c_uint32, C.c_uint32]),
     "glx_init_accel": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _u32p, _dp, _dp, _dp, C.c_double]),
     "glx_kill_encounters": (C.c_int, [C.c_void_p]),
     "glx_partition": (C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p]),
@@ -335,10 +335,10 @@ class Context:
         self._check(self.lib.glx_run_block(self.h, buf, dt, outer_radius, f))
 
     def run_blocks(self, n_blocks: int, first_buf: int, dt: float, outer_radius: float, form: str | int = "fp64",
-                   n_shards: int = 4, waves_per_simd: int = 3) -> None:
+                   n_shards: int = 4, waves_per_simd: int = 3, launches_per_block: int = 1) -> None:
         f = FORMS[form] if isinstance(form, str) else int(form)
         self._check(self.lib.glx_run_blocks(self.h, n_blocks, first_buf, dt, outer_radius, f, n_shards,
-                                            waves_per_simd))
+                                            waves_per_simd, launches_per_block))
 
     def init_accel(self, m, rh, planet_r, h0, outer_radius: float, ids=None) -> None:
         m = np.ascontiguousarray(m, dtype=np.float64)

@@ -149,16 +149,19 @@ int glx_run_block(glx_ctx* ctx, int buf, double dt, double outer_radius, int for
 /* Advance particles [0, n_alive) through n_blocks consecutive planet blocks
  * held in buffers first_buf, first_buf+1, ... (modulo the buffer count),
  * async.  The particles are split into n_shards slices, each run on its own
- * stream as a chain of single-block launches, with no compaction between
- * blocks (flagged particles simply stop).  waves_per_simd > 0 caps the
- * resident waves per SIMD (3..8) through the LDS request, so that a SIMD
- * freed early picks up the next ready block of another slice; 0 leaves
- * occupancy uncapped.  deathtime_index becomes the step count within the
- * launch (k * Time-Block-Size + step + 1).  With n_blocks = 1 and
- * n_shards = 1 it is glx_run_block.  No reference counterpart: the
- * reference launches one block per host iteration (src/executor.cu:640-646). */
+ * stream as a chain of launches, with no compaction between blocks (flagged
+ * particles simply stop).  waves_per_simd > 0 caps the resident waves per
+ * SIMD (3..8) through the LDS request, so that a SIMD freed early picks up
+ * the next ready launch of another slice; 0 leaves occupancy uncapped.
+ * launches_per_block (1..64, 0 means 1) runs each block's steps as that
+ * many consecutive launches (same results; a shorter last launch shortens
+ * the partly idle end of the chain).  deathtime_index becomes the step
+ * count within the launch (k * Time-Block-Size + step + 1).  With n_blocks
+ * = 1, n_shards = 1 and one launch per block it is glx_run_block.  No
+ * reference counterpart: the reference launches one block per host
+ * iteration (src/executor.cu:640-646). */
 int glx_run_blocks(glx_ctx* ctx, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
-                   uint32_t n_shards, uint32_t waves_per_simd);
+                   uint32_t n_shards, uint32_t waves_per_simd, uint32_t launches_per_block);
 
 /* Initial accelerations and flags of particles [0, n_alive) from planet row
  * 0 of `planet_r` ((n_bodies-1) x 3) and h0 (3 doubles), async.  The

@@ -256,11 +256,12 @@ def test_fast_forms_encounter_flags(native, oracle, form):
 
 
 @pytest.mark.parametrize("form", ["fp64", "fast"])
-@pytest.mark.parametrize("shards,cap", [(1, 0), (4, 3), (3, 4), (8, 3)])
-def test_run_blocks_equals_single_blocks(native, oracle, form, shards, cap):
-    """One multi-block launch (sharded, occupancy-capped) == the same blocks
-    launched one by one without compaction in between: bit-identical state
-    and flags; deathtime_index counts steps from the launch start."""
+@pytest.mark.parametrize("shards,cap,parts", [(1, 0, 1), (4, 3, 1), (3, 4, 1), (8, 3, 1), (4, 3, 4), (2, 0, 7)])
+def test_run_blocks_equals_single_blocks(native, oracle, form, shards, cap, parts):
+    """One multi-block launch (sharded, occupancy-capped, each block split into
+    `parts` launches) == the same blocks launched one by one without
+    compaction in between: bit-identical state and flags; deathtime_index
+    counts steps from the launch start."""
     from glisser_amd import systems
     s = systems.three_body(5000, seed=31, a_range=(20.0, 40.0), e_hi=0.6, i_hi_deg=5.0)
     T, K = 256, 4
@@ -279,7 +280,7 @@ def test_run_blocks_equals_single_blocks(native, oracle, form, shards, cap):
                 ctx.upload_planet_block(k, s.m, rh, rl, h0)
             death = np.full(n_alive, -1)
             if multi:
-                ctx.run_blocks(K, 0, 30.0, OUTER, form, shards, cap)
+                ctx.run_blocks(K, 0, 30.0, OUTER, form, shards, cap, parts)
             else:
                 for k in range(K):
                     ctx.run_block(k, 30.0, OUTER, form)
