@@ -335,7 +335,7 @@ int glx_batch_create(int device, const glx_system* systems, uint32_t n_systems, 
         (rc = dalloc(b, &b->stage, 3 * (size_t)n_max)) || (rc = dalloc(b, &b->sys_d, n_systems)) ||
         (rc = dalloc(b, &b->tile_sys, slots / 64 + n_systems)) ||
         (rc = dalloc(b, &b->tile_first, slots / 64 + n_systems)) || (rc = dalloc(b, &b->clock, 1)) ||
-        (rc = dalloc(b, &b->queue, 1)) || (rc = dalloc(b, &b->sum, 1)) || (rc = dalloc(b, &hist, hist_doubles)) ||
+        (rc = dalloc(b, &b->queue, 1)) || (rc = dalloc(b, &b->sum, 1 + glx::kSumParts)) || (rc = dalloc(b, &hist, hist_doubles)) ||
         (rc = dalloc(b, &logs, log_doubles)))
         return bail(rc);
     // Zeroed particle state: padding slots stay flagged-free zeros and never step.

@@ -13,6 +13,8 @@
 // so the arrays keep their device addresses.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "launch.h"
 
 namespace glx {
@@ -220,20 +222,33 @@ __global__ __launch_bounds__(256) void pack_encounters_kernel(ParticleSet s, Rol
 }
 
 // One workgroup sums dti[0, n) into *out (no atomics, nothing to clear).
-__global__ __launch_bounds__(1024) void sum_steps_kernel(const uint32_t* __restrict__ dti, uint32_t n,
-                                                         unsigned long long* out) {
+// Sum of dti over a grid-strided share of [0, n) per workgroup into
+// part[blockIdx.x]; then one workgroup adds the parts into *out (no atomics,
+// nothing to clear).
+__device__ __forceinline__ unsigned long long block_sum_1024(unsigned long long s) {
     __shared__ unsigned long long part[16];
-    unsigned long long s = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += dti[i];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
+    unsigned long long t = 0;
+    if (threadIdx.x == 0)
         for (int w = 0; w < 16; w++) t += part[w];
-        *out = t;
-    }
+    return t;
+}
+
+__global__ __launch_bounds__(1024) void sum_steps_kernel(const uint32_t* __restrict__ dti, uint32_t n,
+                                                         unsigned long long* part) {
+    unsigned long long s = 0;
+    for (uint32_t i = blockIdx.x * 1024 + threadIdx.x; i < n; i += gridDim.x * 1024) s += dti[i];
+    const unsigned long long t = block_sum_1024(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(1024) void sum_parts_kernel(const unsigned long long* __restrict__ part, uint32_t n,
+                                                         unsigned long long* out) {
+    const unsigned long long t = block_sum_1024(threadIdx.x < n ? part[threadIdx.x] : 0);
+    if (threadIdx.x == 0) *out = t;
 }
 
 __global__ __launch_bounds__(256) void aos_to_soa_kernel(const double* __restrict__ aos, double* x, double* y,
@@ -304,7 +319,9 @@ hipError_t launch_pack_encounters(ParticleSet cur, const RollbackSet* rb, uint32
 }
 
 hipError_t launch_sum_steps(const uint32_t* dti, uint32_t n, unsigned long long* out, hipStream_t stream) {
-    hipLaunchKernelGGL(sum_steps_kernel, dim3(1), dim3(1024), 0, stream, dti, n, out);
+    const uint32_t groups = std::max(1u, std::min(kSumParts, (n + 8191) / 8192));
+    hipLaunchKernelGGL(sum_steps_kernel, dim3(groups), dim3(1024), 0, stream, dti, n, out + 1);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(1024), 0, stream, out + 1, groups, out);
     return hipGetLastError();
 }
 

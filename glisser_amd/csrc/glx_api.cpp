@@ -291,7 +291,7 @@ int glx_create(const glx_params* p, glx_ctx** out) {
     if (c->keep_rb && ((rc = alloc_rb(c, &c->rb)) || (rc = alloc_rb(c, &c->tmp_rb)))) return bail(rc);
     const size_t tiles = glx::part_tiles(c->cap);
     if ((rc = dalloc(c, &c->counts, 2 * tiles)) || (rc = dalloc(c, &c->offsets, 2 * tiles)) ||
-        (rc = dalloc(c, &c->totals, 2)) || (rc = dalloc(c, &c->sum_dev, 1)) ||
+        (rc = dalloc(c, &c->totals, 2)) || (rc = dalloc(c, &c->sum_dev, 1 + glx::kSumParts)) ||
         (rc = dalloc(c, &c->dev_aos, 3 * (c->cap > GLX_MAX_BODIES ? c->cap : GLX_MAX_BODIES))))
         return bail(rc);
     if (hipHostMalloc((void**)&c->pin_small, 64, hipHostMallocDefault) != hipSuccess)

@@ -42,6 +42,9 @@ hipError_t launch_partition(ParticleSet cur, ParticleSet tmp, const RollbackSet*
 
 hipError_t launch_kill_encounters(uint16_t* flags, uint32_t n, hipStream_t stream);
 hipError_t launch_snapshot(ParticleSet cur, RollbackSet rb, uint32_t n, hipStream_t stream);
+// Sum of dti[0, n) into out[0]; out holds 1 + kSumParts entries (the rest
+// are the workgroups' partial sums).
+constexpr uint32_t kSumParts = 256;
 hipError_t launch_sum_steps(const uint32_t* dti, uint32_t n, unsigned long long* out, hipStream_t stream);
 // Host close-encounter records of particles [begin, begin + n); rb may be null.
 hipError_t launch_pack_encounters(ParticleSet cur, const RollbackSet* rb, uint32_t begin, uint32_t n, double t_block,
