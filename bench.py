@@ -47,7 +47,7 @@ HEADLINE_FORM = "fast"
 # summed over the block kernel and divided by its particle-steps); see
 # profiles/wh_block_200k_counters.md.  Used with the live rate to report
 # the counter-based share of FP64 vector peak.
-COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 322.0, "mixed": 298.0}
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 287.0, "mixed": 283.0}
 
 # HBM bytes per time-block of the fast form at this workload, from
 # rocprofv3 FETCH_SIZE (x 2.0, the factor a copy kernel of known size needs)

@@ -6,9 +6,10 @@
 //   * one reciprocal square root of |r|^2 and of 1/a replace the drift's
 //     divisions and square roots; 1/a = -2E/mu needs no division at all;
 //   * reciprocals / reciprocal square roots start from v_rcp_f64 /
-//     v_rsq_f64 (relative error < 6e-8) and take two FMA Newton steps where
-//     the result feeds the state, none where it only scales a Newton
-//     correction;
+//     v_rsq_f64 (relative error < 6e-8) and take one FMA Newton step where
+//     the result feeds the state (relative error < 6e-15, a few ulp of the
+//     values it scales, far below the step's truncation error), none where
+//     it only scales a Newton correction;
 //   * sin/cos of the mean-anomaly step come from a five-term Taylor
 //     polynomial (|dM| <= 0.2); the first guess and each Newton correction
 //     rotate (sin E, cos E - 1) by the small angle with a short series
@@ -33,30 +34,19 @@
 
 namespace glx {
 
-// Reciprocal of x (x normal, positive or negative), two Newton steps from
-// the hardware estimate: faithful to about one ulp.
-__device__ __forceinline__ double rcp_nr(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-x, y, 1.0);
-    return fma(y, e, y);
-}
-
-// Reciprocal with one Newton step (relative error below 3e-15).
+// Reciprocal of x (x normal) with one Newton step from v_rcp_f64: the
+// estimate's relative error (< 6e-8) is squared, leaving < 3e-15 (measured
+// on gfx950: 2.2e-15).
 __device__ __forceinline__ double rcp_nr1(double x) {
     const double y = __builtin_amdgcn_rcp(x);
     return fma(y, fma(-x, y, 1.0), y);
 }
 
-// 1/sqrt(x) for x > 0, two Newton steps from the hardware estimate.
-__device__ __forceinline__ double rsqrt_nr(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-    double e = fma(-h, y * y, 0.5);
-    y = fma(y, e, y);
-    e = fma(-h, y * y, 0.5);
-    return fma(y, e, y);
+// 1/sqrt(x) for x > 0 with one Newton step from v_rsq_f64: relative error
+// about 1.5 times the estimate's squared, < 6e-15 (measured: 4.3e-15).
+__device__ __forceinline__ double rsqrt_nr1(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    return fma(y, fma(-0.5 * x, y * y, 0.5), y);
 }
 
 // Taylor coefficients of sin (x^3..x^11) and of cos - 1 (x^2..x^10):
@@ -139,11 +129,11 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     const double r2 = p.r2;  // |r|^2, left by the previous step's planet sum
     const double v2 = fma(vz0, vz0, fma(vy0, vy0, vx0 * vx0));
     const double vdotr = fma(vz0, rz0, fma(vy0, ry0, vx0 * rx0));
-    const double idist = rsqrt_nr(r2);
+    const double idist = rsqrt_nr1(r2);
     const double dist = r2 * idist;
     // 1/a = -2E/mu = 2/|r| - v^2/mu; bound orbits have 1/a > 0
     const double ia = fma(-v2, inv_mu, 2.0 * idist);
-    const double rs_ia = rsqrt_nr(ia);              // sqrt(a)
+    const double rs_ia = rsqrt_nr1(ia);              // sqrt(a)
     const double a = rs_ia * rs_ia;
     const double n_ = sqrt_mu * ia * (ia * rs_ia);  // sqrt(mu / a^3)
     const double in_ = inv_sqrt_mu * a * rs_ia;     // 1/n
@@ -178,7 +168,7 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     bad |= !conv;
     const double dE = dM + dd;
     fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
-    const double ifp = rcp_nr(fp);
+    const double ifp = rcp_nr1(fp);
     const double fg = fma(a * cm1, idist, 1.0);
     const double g = fma(s - dE, in_, dt);
     const double fdot = -n_ * s * a * idist * ifp;
@@ -205,12 +195,12 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
 
 // Planet sum and flag tests, FAST and MIXED forms.
 //
-// r^-3 of each planet: FAST takes 1/|dr| from v_rsq_f64 with two FP64
-// Newton steps; MIXED starts from the FP32 reciprocal square root (v_rsq_f32,
-// ~1e-7) and takes one FP64 Newton step (error ~1e-14, below the rounding
-// of the FP64 sum), trading the FP64 transcendental and one correction for
-// two conversions.  The terms, their accumulation, |dr|^2 and every test
-// are FP64.
+// r^-3 of each planet: FAST takes 1/|dr| from v_rsq_f64 with one FP64
+// Newton step (error < 6e-15); MIXED starts from the FP32 reciprocal square
+// root (v_rsq_f32, ~1e-7) and takes the same FP64 Newton step (error
+// ~2e-14, still below the planet term's share of the step error), trading
+// the FP64 transcendental for two conversions.  The terms, their
+// accumulation, |dr|^2 and every test are FP64.
 //
 // The encounter, Sun and outer-boundary tests keep the reference's
 // precedence (src/wh.cu:151-170: the first planet inside its radius wins if
@@ -238,7 +228,7 @@ __device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict
             y = (double)__builtin_amdgcn_rsqf((float)rad);
             y = fma(y, fma(-0.5 * rad, y * y, 0.5), y);
         } else {
-            y = rsqrt_nr(rad);
+            y = rsqrt_nr1(rad);
         }
         const double fac = pc.m[j] * (y * y * y);
         ax = fma(-dx, fac, ax);
