@@ -38,6 +38,12 @@ _SIGS = {
     "stable_partition": (C.c_uint64, [_u16p, C.c_uint64, C.c_uint64, C.c_int, _u64p]),
     "to_elements": (None, [C.c_double, _dp, _dp, _dp]),
     "from_elements": (None, [C.c_double, _dp, _dp, _dp]),
+    "from_elements_mean": (None, [C.c_double, _dp, _dp, _dp]),
+    "plhist_write": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_uint32, _dp, _dp, _dp, _u32p]),
+    "plhist_open": (C.c_void_p, [C.c_char_p, C.c_double, C.c_double, C.c_uint32]),
+    "plhist_free": (None, [C.c_void_p]),
+    "plhist_state": (None, [C.c_void_p, _u32p, _dp, _u32p, _dp, _dp]),
+    "plhist_block": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _dp, _u32p, _u32p, _dp, _u32p, _dp, _dp]),
 }
 _REF_ONLY = {
     "config_normalize": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint64, _u64p]),
@@ -237,6 +243,26 @@ class Oracle:
         self._f("from_elements")(mu, _p(el, _dp), _p(r, _dp), _p(v, _dp))
         return r, v
 
+    def from_elements_mean(self, mu, el):
+        el = np.ascontiguousarray(el, np.float64)
+        r, v = np.empty(3), np.empty(3)
+        self._f("from_elements_mean")(mu, _p(el, _dp), _p(r, _dp), _p(v, _dp))
+        return r, v
+
+    # -- planet history file ---------------------------------------------
+    def plhist_write(self, path, time: float, m, r, v, ids=None, append=False) -> None:
+        m = np.ascontiguousarray(m, np.float64)
+        r = np.ascontiguousarray(r, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        ids = np.arange(len(m), dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, np.uint32)
+        rc = self._f("plhist_write")(str(path).encode(), int(append), time, len(m), _p(m, _dp), _p(r, _dp),
+                                     _p(v, _dp), _p(ids, _u32p))
+        if rc != 0:
+            raise OracleError(rc, self.err())
+
+    def plhist(self, path, t0: float, dt: float, maxpl: int = 16):
+        return OracleHistory(self, path, t0, dt, maxpl)
+
 
 def _state_arrays(st: dict) -> dict:
     n = len(st["id"])
@@ -284,3 +310,47 @@ class OraclePlanets:
 
     def recalculate_rh(self):
         self.o._f("planets_recalculate_rh")(self.h)
+
+
+class OracleHistory:
+    """Planet history file driven as the reference's executor drives its
+    Interpolator: state() is the planet table, block(tbsize) the next block."""
+
+    def __init__(self, o: Oracle, path, t0, dt, maxpl):
+        self.o = o
+        self.maxpl = maxpl
+        self.h = o._f("plhist_open")(str(path).encode(), t0, dt, maxpl)
+        if not self.h:
+            raise OracleError(3, o.err())
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.o._f("plhist_free")(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def state(self):
+        nb = np.zeros(1, np.uint32)
+        m, ids = np.empty(self.maxpl), np.empty(self.maxpl, np.uint32)
+        r, v = np.empty((self.maxpl, 3)), np.empty((self.maxpl, 3))
+        self.o._f("plhist_state")(self.h, _p(nb, _u32p), _p(m, _dp), _p(ids, _u32p), _p(r, _dp), _p(v, _dp))
+        return {"n_bodies": int(nb[0]), "m": m, "id": ids, "r": r, "v": v}
+
+    def block(self, tbsize: int):
+        t = np.zeros(1)
+        dt = np.zeros(1)
+        ns = np.zeros(1, np.uint32)
+        nb = np.zeros(1, np.uint32)
+        m, ids = np.empty(self.maxpl), np.empty(self.maxpl, np.uint32)
+        rl = np.empty((tbsize, self.maxpl - 1, 3))
+        h0 = np.empty((tbsize, 3))
+        rc = self.o._f("plhist_block")(self.h, tbsize, _p(t, _dp), _p(dt, _dp), _p(ns, _u32p), _p(nb, _u32p),
+                                       _p(m, _dp), _p(ids, _u32p), _p(rl, _dp), _p(h0, _dp))
+        if rc != 0:
+            raise OracleError(rc, self.o.err())
+        n, b = int(ns[0]), int(nb[0])
+        r_log = rl.reshape(-1)[: n * (b - 1) * 3].reshape(n, b - 1, 3)
+        return {"t": float(t[0]), "dt": float(dt[0]), "n_steps": n, "n_bodies": b, "m": m[:b].copy(),
+                "id": ids[:b].copy(), "r_log": r_log.copy(), "h0_log": h0[:n].copy()}

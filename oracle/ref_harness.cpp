@@ -9,11 +9,15 @@
 // (src/wh.cpp:498-577), integrate_planets_timeblock (src/wh.cpp:584-595),
 // integrate_particles_timeblock (src/wh.cpp:597-609), load_h0
 // (src/wh.cpp:993-1025) and HostParticlePhaseSpace::stable_partition_*
-// (src/data.cpp:105-123).
+// (src/data.cpp:105-123); for planet history files the Interpolator
+// (src/interp.cpp) driven the way Executor::init / update_planets drive it
+// (src/executor.cu:93-130,235-292,700; that file is CUDA, so its few lines
+// of glue are repeated here) and the writer save_swift_plhist.
 
 #include "wh.h"
 #include "convert.h"
 #include "data.h"
+#include "interp.h"
 
 #include <cstdint>
 #include <cstring>
@@ -377,6 +381,116 @@ int ref_track_write(const char* path, int append, double time, uint32_t np, cons
     std::ofstream o(path, std::ios_base::binary | (append ? std::ios_base::app : std::ios_base::trunc));
     save_binary_track(o, pl, pa, time, to_elements != 0, barycentric != 0);
     return 0;
+}
+
+// ---- planet history (src/interp.cpp, src/data.cpp:817-873) ------------------
+
+int ref_plhist_write(const char* path, int append, double time, uint32_t n, const double* m, const double* r,
+                     const double* v, const uint32_t* id) {
+    HostPlanetSnapshot pl;
+    fill_planets(pl, n, m, r, v, id);
+    std::ofstream o(path, std::ios_base::binary | (append ? std::ios_base::app : std::ios_base::trunc));
+    if (!append) begin_swift_plhist(o, pl);
+    save_swift_plhist(o, pl, time);
+    return o ? 0 : 3;
+}
+
+namespace {
+struct RefHistory {
+    Configuration cfg;
+    HostPlanetPhaseSpace pl;
+    sr::interp::Interpolator* interp = nullptr;
+    WHIntegrator integ;
+    double t = 0;
+    ~RefHistory() { delete interp; }
+};
+}  // namespace
+
+void* ref_plhist_open(const char* path, double t0, double dt, uint32_t maxpl) {
+    RefHistory* h = new RefHistory();
+    try {
+        h->cfg.t_0 = t0;
+        h->cfg.dt = dt;
+        h->cfg.interp_maxpl = maxpl;
+        h->cfg.tbsize = 1;
+        h->interp = new sr::interp::Interpolator(h->cfg, h->pl, path);
+        sr::interp::Interpolator& in = *h->interp;
+        h->t = t0;
+        while (in.t1 <= h->t) in.next(h->pl);
+        if (std::isinf(in.t0)) throw std::runtime_error("lookup file starts past the given Initial-Time");
+        in.cur_ts = 0;
+        in.n_ts = std::max<size_t>(1, static_cast<size_t>(std::round((in.t1 - h->t) / dt)));
+        in.eff_dt = (in.t1 - h->t) / static_cast<double>(in.n_ts);
+        in.rel_t = h->t - in.t0;
+        in.fill_one(h->pl, in.rel_t);
+    } catch (const std::exception& e) {
+        std::snprintf(g_err, sizeof g_err, "%s", e.what());
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void ref_plhist_free(void* h) { delete static_cast<RefHistory*>(h); }
+
+void ref_plhist_state(void* hp, uint32_t* n_bodies, double* m, uint32_t* id, double* r, double* v) {
+    RefHistory& h = *static_cast<RefHistory*>(hp);
+    if (n_bodies) *n_bodies = static_cast<uint32_t>(h.pl.n_alive());
+    for (size_t k = 0; k < h.pl.n(); k++) {
+        if (m) m[k] = h.pl.m()[k];
+        if (id) id[k] = h.pl.id()[k];
+        if (r) st(r, k, h.pl.r()[k]);
+        if (v) st(v, k, h.pl.v()[k]);
+    }
+}
+
+int ref_plhist_block(void* hp, uint32_t tbsize, double* t_start, double* dt, uint32_t* n_steps, uint32_t* n_bodies,
+                     double* m, uint32_t* id, double* r_log, double* h0_log) {
+    RefHistory& h = *static_cast<RefHistory*>(hp);
+    sr::interp::Interpolator& in = *h.interp;
+    try {
+        if (in.cur_ts == in.n_ts) {
+            double diff = in.rel_t - (in.t1 - in.t0);
+            if (!(std::abs(diff) < 1e-8)) throw std::runtime_error("sanity check failed - adjusting t by too much");
+            in.next(h.pl);
+            in.cur_ts = 0;
+            h.t = in.t0;
+        }
+        const double cur_dt = in.eff_dt;
+        const uint32_t n = std::min(tbsize, static_cast<uint32_t>(in.n_ts - in.cur_ts));
+        if (h.pl.r_log().old.size() < (h.pl.n() - 1) * n) {
+            h.pl.r_log() = sr::util::History<Vf64_3>((h.pl.n() - 1) * n);
+            h.pl.v_log() = sr::util::History<Vf64_3>((h.pl.n() - 1) * n);
+        }
+        in.fill(h.pl, n, in.rel_t, cur_dt);
+        in.rel_t += static_cast<double>(n) * cur_dt;
+        in.cur_ts += n;
+        if (h.integ.planet_h0_log.old.size() < n) h.integ.planet_h0_log = sr::util::History<Vf64_3>(n);
+        h.integ.load_h0(h.pl);
+        *t_start = h.t;
+        *dt = cur_dt;
+        *n_steps = n;
+        *n_bodies = static_cast<uint32_t>(h.pl.n_alive());
+        for (size_t k = 0; k < h.pl.n_alive(); k++) {
+            m[k] = h.pl.m()[k];
+            id[k] = h.pl.id()[k];
+        }
+        for (size_t k = 0; k < static_cast<size_t>(n) * (h.pl.n_alive() - 1); k++) st(r_log, k, h.pl.r_log().old[k]);
+        if (h0_log)
+            for (uint32_t k = 0; k < n; k++) st(h0_log, k, h.integ.planet_h0_log.old[k]);
+        h.t += cur_dt * static_cast<double>(n);
+    } catch (const std::exception& e) {
+        std::snprintf(g_err, sizeof g_err, "%s", e.what());
+        return 3;
+    }
+    return 0;
+}
+
+void ref_from_elements_mean(double mu, const double* el6, double* r, double* v) {
+    f64_3 R, V;
+    sr::convert::from_elements_M(mu, el6[0], el6[1], el6[2], el6[3], el6[4], el6[5], &R, &V);
+    st(r, 0, R);
+    st(v, 0, V);
 }
 
 }  // extern "C"

@@ -34,7 +34,12 @@
 #include <cstdio>
 #include <vector>
 #include <algorithm>
+#include <fstream>
+#include <limits>
 #include <numeric>
+
+extern "C" void orc_load_h0(uint32_t n_bodies, const double* m, const double* r_log, uint32_t n_steps,
+                            double* h0_log);
 
 namespace {
 
@@ -463,6 +468,391 @@ int step_planets(Planets& p, double dt, V3* r_row, V3* v_row, V3* h0_out, size_t
 
 thread_local char g_err[256];
 
+// ---- planet history file: element-linear interpolation --------------------
+
+// Eccentric anomaly from mean anomaly: ehie for e >= 0.8, eget<1|2> below
+// (src/convert.cpp:156-228).  Local sin/cos pairs are left for the compiler
+// to fuse, as in the reference build.
+double ehie(double e, double m) {
+    int iflag = 0;
+    int nper = static_cast<int>(m / (2 * kPi));
+    m = m - nper * 2 * kPi;
+    if (m < 0) m += 2 * kPi;
+    if (m > kPi) {
+        m = 2 * kPi - m;
+        iflag = 1;
+    }
+    double x = std::pow(6 * m, 1. / 3) - m;
+    for (int it = 0; it < 3; it++) {
+        double sa = std::sin(x + m);
+        double ca = std::cos(x + m);
+        double esa = e * sa;
+        double eca = e * ca;
+        double f = x - esa;
+        double fp = 1 - eca;
+        double dx = -f / fp;
+        dx = -f / (fp + dx * esa / 2);
+        dx = -f / (fp + dx * (esa + eca * dx / 3) / 2);
+        x = x + dx;
+    }
+    return iflag ? 2 * kPi - m - x : m + x;
+}
+
+template <int N>
+double eget(double e, double m) {
+    double sm = std::sin(m);
+    double cm = std::cos(m);
+    double x = m + e * sm * (1 + e * (cm + e * (1 - 1.5 * sm * sm)));
+    for (int it = 0; it < N; it++) {
+        double sx = std::sin(x);
+        double cx = std::cos(x);
+        double es = e * sx;
+        double ec = e * cx;
+        double f = x - es - m;
+        double fp = 1 - ec;
+        double fpp = es;
+        double fppp = ec;
+        double dx = -f / fp;
+        dx = -f / (fp + dx * fpp / 2);
+        dx = -f / (fp + dx * fpp / 2 + dx * dx * fppp / 6);
+        x = x + dx;
+    }
+    return x;
+}
+
+double ehybrid(double e, double m) {
+    if (e < 0.18) return eget<1>(e, m);
+    if (e < 0.8) return eget<2>(e, m);
+    return ehie(e, m);
+}
+
+// from_elements_M (src/convert.cpp:230-248): true anomaly from the mean
+// anomaly, sign taken from E reduced to (-pi, pi].
+void from_elements_M(double mu, double a, double e, double i, double capom, double om, double M, V3* r, V3* v) {
+    double E = ehybrid(e, M);
+    double f = std::acos((std::cos(E) - e) / (1 - e * std::cos(E)));
+    E = E - static_cast<int>(E / 2 / kPi) * 2 * kPi;
+    if (E > kPi)
+        E -= 2 * kPi;
+    else if (E < -kPi)
+        E += 2 * kPi;
+    if (E < 0) f = -f;
+    from_elements(mu, a, e, i, capom, om, f, r, v);
+}
+
+// get_mean_anomaly (src/convert.cpp:465-470).
+double mean_anomaly(double e, double f) {
+    double E = std::acos((e + std::cos(f)) / (1 + e * std::cos(f)));
+    if (f < 0) E = -E;
+    return E - e * std::sin(E);
+}
+
+template <typename T>
+T rd(std::istream& in) {
+    T t;
+    in.read(reinterpret_cast<char*>(&t), sizeof(T));
+    return t;
+}
+
+// The Interpolator (src/interp.cpp:13-293) together with the executor glue
+// that drives it (src/executor.cu:93-130 at start-up, :235-292 per block,
+// :700 time advance).  Slot layout follows the reference's planet phase
+// space of interp_maxpl entries: slot 0 the Sun, slot k the planet whose id
+// was k-th to appear in the file.
+struct History {
+    std::ifstream in;
+    double user_dt = 0;
+    size_t nmax = 0;
+    std::vector<double> pm;   // pl.m()
+    std::vector<uint32_t> pid;  // pl.id()
+    std::vector<V3> pr, pv;   // pl.r(), pl.v()
+    size_t pl_alive = 0;
+    std::vector<V3> aei0, aei1, oom0, oom1;
+    std::vector<double> m1;
+    double t0 = 0, t1 = 0;
+    size_t npl1 = 0;
+    std::vector<V3> rdaei, rdoom, raei_i, room_i;
+    std::vector<uint32_t> rids;
+    std::vector<double> rm;
+    size_t n_alive = 0;
+    std::vector<std::pair<uint32_t, size_t>> idmap;  // insertion order = slot
+    size_t n_ts = 0, cur_ts = 0;
+    double eff_dt = 0, rel_t = 0;
+    double t = 0;  // executor time at the start of the next block
+
+    // idmap lookup; a new id takes the next slot (the map's size before insertion)
+    bool slot_of(uint32_t id, size_t* ind, bool* fresh) {
+        for (auto& kv : idmap)
+            if (kv.first == id) {
+                *ind = kv.second;
+                *fresh = false;
+                return true;
+            }
+        if (idmap.size() >= nmax) return false;
+        *ind = idmap.size();
+        idmap.emplace_back(id, *ind);
+        *fresh = true;
+        return true;
+    }
+
+    // Interpolator::Interpolator (src/interp.cpp:13-91)
+    int open(const char* path, double t_start, double dt, size_t maxpl) {
+        in.open(path, std::ios_base::binary);
+        if (!in) {
+            std::snprintf(g_err, sizeof g_err, "cannot open %s", path);
+            return kBadInput;
+        }
+        user_dt = dt;
+        nmax = maxpl;
+        pm.assign(nmax, 0);
+        pid.assign(nmax, 0);
+        pr.assign(nmax, v3(0, 0, 0));
+        pv.assign(nmax, v3(0, 0, 0));
+        pm[0] = rd<double>(in);
+        in.seekg(32 - 8, std::ios_base::cur);
+        aei0 = aei1 = oom0 = oom1 = std::vector<V3>(nmax, v3(0, 0, 0));
+        m1.assign(nmax, 0);
+        rm.assign(nmax, 0);
+        rdaei = rdoom = raei_i = room_i = std::vector<V3>(nmax, v3(0, 0, 0));
+        rids.assign(nmax, 0);
+        t0 = HUGE_VAL;
+        t1 = rd<double>(in);
+        npl1 = rd<uint32_t>(in) + 1;
+        pl_alive = npl1;
+        in.seekg(32 - 8 - 4, std::ios_base::cur);
+        if (t1 > t_start) {
+            std::snprintf(g_err, sizeof g_err, "time t0 not found in datafile: planet history starts after t0");
+            return kBadInput;
+        }
+        idmap.clear();
+        idmap.emplace_back(0u, 0);
+        for (size_t i = 1; i < npl1; i++) {
+            uint32_t id = rd<uint32_t>(in);
+            size_t ind;
+            bool fresh;
+            if (!slot_of(id, &ind, &fresh) || i >= nmax) {
+                std::snprintf(g_err, sizeof g_err, "planet history holds more planets than the table (%zu)", nmax);
+                return kBadInput;
+            }
+            if (fresh) pid[ind] = id;
+            m1[ind] = rd<float>(in);
+            aei1[ind].x = rd<float>(in);
+            aei1[ind].y = rd<float>(in);
+            aei1[ind].z = rd<float>(in);
+            oom1[ind].x = rd<float>(in);
+            oom1[ind].y = rd<float>(in);
+            oom1[ind].z = rd<float>(in);
+            V3 r, v;
+            from_elements_M(pm[0] + m1[ind], aei1[ind].x, aei1[ind].y, aei1[ind].z, oom1[ind].x, oom1[ind].y,
+                            oom1[ind].z, &r, &v);
+            pr[i] = r;
+            pv[i] = v;
+            pid[i] = id;
+            pm[i] = m1[ind];
+        }
+        pr[0] = v3(0, 0, 0);
+        pv[0] = v3(0, 0, 0);
+        aei1[0] = v3(0, 0, 0);
+        oom1[0] = v3(0, 0, 0);
+        // Executor::init (src/executor.cu:99-129)
+        t = t_start;
+        while (t1 <= t) {
+            int st = next();
+            if (st != kOk) return st;
+        }
+        if (std::isinf(t0)) {
+            std::snprintf(g_err, sizeof g_err, "lookup file starts past the given Initial-Time");
+            return kBadInput;
+        }
+        cur_ts = 0;
+        n_ts = std::max<size_t>(1, static_cast<size_t>(std::round((t1 - t) / dt)));
+        eff_dt = (t1 - t) / static_cast<double>(n_ts);
+        rel_t = t - t0;
+        fill_one(rel_t);
+        return kOk;
+    }
+
+    // Interpolator::next (src/interp.cpp:167-293)
+    int next() {
+        aei0 = aei1;
+        oom0 = oom1;
+        t0 = t1;
+        rel_t = 0;
+        t1 = rd<double>(in);
+        if (!in) {
+            std::snprintf(g_err, sizeof g_err, "EOS reached for lookup file.");
+            return kBadInput;
+        }
+        npl1 = rd<uint32_t>(in) + 1;
+        double dt = t1 - t0;
+        n_ts = std::max<size_t>(1, static_cast<size_t>(std::round(dt / user_dt)));
+        eff_dt = dt / static_cast<double>(n_ts);
+        in.seekg(32 - 8 - 4, std::ios_base::cur);
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        for (size_t i = 1; i < nmax; i++) {
+            aei1[i] = v3(nan, nan, nan);
+            oom1[i] = v3(nan, nan, nan);
+        }
+        size_t k = 0;
+        for (size_t i = 1; i < npl1; i++) {
+            uint32_t id = rd<uint32_t>(in);
+            size_t ind;
+            bool fresh;
+            if (!slot_of(id, &ind, &fresh)) {
+                std::snprintf(g_err, sizeof g_err, "planet history holds more planets than the table (%zu)", nmax);
+                return kBadInput;
+            }
+            if (fresh) pid[ind] = id;
+            pm[ind] = rd<float>(in);
+            aei1[ind].x = rd<float>(in);
+            aei1[ind].y = rd<float>(in);
+            aei1[ind].z = rd<float>(in);
+            oom1[ind].x = rd<float>(in);
+            oom1[ind].y = rd<float>(in);
+            oom1[ind].z = rd<float>(in);
+            if (std::isnan(aei0[ind].x)) continue;
+            rids[k] = id;
+            rm[k] = m1[ind];
+            raei_i[k] = aei0[ind];
+            room_i[k] = oom0[ind];
+            rdaei[k] = divs(sub(aei1[ind], aei0[ind]), dt);
+            V3 doom = sub(oom1[ind], oom0[ind]);
+            if (doom.x > kPi)
+                doom.x -= 2 * kPi;
+            else if (doom.x < -kPi)
+                doom.x += 2 * kPi;
+            if (doom.y > kPi)
+                doom.y -= 2 * kPi;
+            else if (doom.y < -kPi)
+                doom.y += 2 * kPi;
+            doom = divs(doom, dt);
+            double mmfreq = 2 * kPi * std::sqrt(1 + pm[ind] / pm[0]) *
+                            (std::pow(aei0[ind].x, -1.5) + std::pow(aei1[ind].x, -1.5)) / 2;
+            double cmfin = oom0[ind].z + mmfreq * dt;
+            cmfin = std::fmod(cmfin, 2 * kPi);
+            double corr = oom1[ind].z - cmfin;
+            if (corr > kPi)
+                corr -= 2 * kPi;
+            else if (corr < -kPi)
+                corr += 2 * kPi;
+            mmfreq += corr / dt;
+            doom.z = mmfreq;
+            rdoom[k] = doom;
+            k++;
+        }
+        n_alive = k;
+        return kOk;
+    }
+
+    // one planet's heliocentric state at relative time rt
+    void state_at(size_t j, double rt, V3* r, V3* v) const {
+        V3 aei = add(raei_i[j], scale(rdaei[j], rt));
+        V3 oom = add(room_i[j], scale(rdoom[j], rt));
+        double gm = rm[j] + pm[0];
+        from_elements_M(gm, aei.x, aei.y, aei.z, oom.x, oom.y, oom.z, r, v);
+    }
+
+    // Interpolator::fill_one (src/interp.cpp:93-111)
+    void fill_one(double rt) {
+        pl_alive = n_alive + 1;
+        for (size_t j = 0; j < n_alive; j++) state_at(j, rt, &pr[j + 1], &pv[j + 1]);
+    }
+
+    // Executor::update_planets for an interpolated history
+    // (src/executor.cu:235-292) with Interpolator::fill (src/interp.cpp:113-165)
+    // and the time advance of src/executor.cu:700.
+    int block(uint32_t tbsize, double* t_start, double* dt_out, uint32_t* n_steps, uint32_t* n_bodies, double* m,
+              uint32_t* id, double* r_log, double* h0_log) {
+        if (cur_ts == n_ts) {
+            double diff = rel_t - (t1 - t0);
+            if (!(std::abs(diff) < 1e-8)) {
+                std::snprintf(g_err, sizeof g_err, "sanity check failed - adjusting t by too much: %g", diff);
+                return kBadInput;
+            }
+            int st = next();
+            if (st != kOk) return st;
+            cur_ts = 0;
+            t = t0;
+        }
+        const double cur_dt = eff_dt;
+        const uint32_t n = std::min(tbsize, static_cast<uint32_t>(n_ts - cur_ts));
+        double rt = rel_t + cur_dt;
+        pl_alive = n_alive + 1;
+        for (size_t j = 0; j < n_alive; j++) {
+            pid[j + 1] = rids[j];
+            pm[j + 1] = rm[j];
+        }
+        for (uint32_t s = 0; s < n; s++) {
+            if (rt > t1 - t0 + 1e-8) {
+                std::snprintf(g_err, sizeof g_err, "interpolation ran too long, timechunk exceeded lookup boundary");
+                return kBadInput;
+            }
+            pr[0] = v3(0, 0, 0);
+            pv[0] = v3(0, 0, 0);
+            for (size_t j = 0; j < n_alive; j++) {
+                state_at(j, rt, &pr[j + 1], &pv[j + 1]);
+                store3(r_log, (size_t)s * n_alive + j, pr[j + 1]);
+            }
+            rt += cur_dt;
+        }
+        *t_start = t;
+        *dt_out = cur_dt;
+        *n_steps = n;
+        *n_bodies = static_cast<uint32_t>(pl_alive);
+        for (size_t k = 0; k < pl_alive; k++) {
+            m[k] = pm[k];
+            id[k] = pid[k];
+        }
+        if (h0_log) orc_load_h0(static_cast<uint32_t>(pl_alive), pm.data(), r_log, n, h0_log);
+        rel_t += static_cast<double>(n) * cur_dt;
+        cur_ts += n;
+        t += cur_dt * static_cast<double>(n);
+        return kOk;
+    }
+};
+
+// save_swift_plhist (src/data.cpp:817-873): one record of heliocentric
+// planets (slot 0 the Sun) as float32 elements with the mean anomaly.
+int write_history(const char* path, int append, double time, uint32_t n, const double* m, const double* r,
+                  const double* v, const uint32_t* id) {
+    std::ofstream out(path, append ? (std::ios_base::binary | std::ios_base::app) : std::ios_base::binary);
+    if (!out) {
+        std::snprintf(g_err, sizeof g_err, "cannot open %s", path);
+        return kBadInput;
+    }
+    auto wr = [&](const auto& x) { out.write(reinterpret_cast<const char*>(&x), sizeof(x)); };
+    auto pad = [&](size_t k) {
+        for (size_t i = 0; i < k; i++) out.put(0);
+    };
+    if (!append) {
+        wr(static_cast<double>(m[0]));
+        pad(32 - 8);
+    }
+    wr(static_cast<double>(time));
+    wr(static_cast<uint32_t>(n > 0 ? n - 1 : 0));
+    pad(32 - 8 - 4);
+    for (uint32_t i = 1; i < n; i++) {
+        double center_mass = m[0];
+        V3 center_r = scale(load3(r, 0), m[0]);
+        V3 center_v = scale(load3(v, 0), m[0]);
+        center_r = divs(center_r, center_mass);
+        center_v = divs(center_v, center_mass);
+        double a, e, in, capom, om, f;
+        to_elements(m[i] + center_mass, sub(load3(r, i), center_r), sub(load3(v, i), center_v), nullptr, &a, &e,
+                    &in, &capom, &om, &f);
+        wr(static_cast<uint32_t>(id[i]));
+        wr(static_cast<float>(m[i]));
+        wr(static_cast<float>(a));
+        wr(static_cast<float>(e));
+        wr(static_cast<float>(in));
+        wr(static_cast<float>(capom));
+        wr(static_cast<float>(om));
+        wr(static_cast<float>(mean_anomaly(e, f)));
+    }
+    out.flush();
+    return out ? kOk : kBadInput;
+}
+
 }  // namespace
 
 extern "C" {
@@ -757,6 +1147,57 @@ void orc_from_elements(double mu, const double* el6, double* r, double* v) {
     from_elements(mu, el6[0], el6[1], el6[2], el6[3], el6[4], el6[5], &R, &Vv);
     store3(r, 0, R);
     store3(v, 0, Vv);
+}
+
+// from_elements_M (mean anomaly in el6[5]).
+void orc_from_elements_mean(double mu, const double* el6, double* r, double* v) {
+    V3 R, Vv;
+    from_elements_M(mu, el6[0], el6[1], el6[2], el6[3], el6[4], el6[5], &R, &Vv);
+    store3(r, 0, R);
+    store3(v, 0, Vv);
+}
+
+// ---- planet history file ----------------------------------------------------
+
+// Start (append = 0) or extend a planet history file with one record of n
+// heliocentric bodies (slot 0 the Sun): begin_swift_plhist +
+// save_swift_plhist (src/data.cpp:817-873).
+int orc_plhist_write(const char* path, int append, double time, uint32_t n, const double* m, const double* r,
+                     const double* v, const uint32_t* id) {
+    return write_history(path, append, time, n, m, r, v, id);
+}
+
+// Open a history for a run starting at t0 with Time-Step dt and a planet
+// table of maxpl slots; NULL on error (message in orc_last_error).
+void* orc_plhist_open(const char* path, double t0, double dt, uint32_t maxpl) {
+    History* h = new History();
+    if (h->open(path, t0, dt, maxpl) != kOk) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void orc_plhist_free(void* h) { delete static_cast<History*>(h); }
+
+// The planet table as the reference's phase space holds it (maxpl slots):
+// *n_bodies alive entries, GM, ids, heliocentric r and v (any output may be NULL).
+void orc_plhist_state(void* hp, uint32_t* n_bodies, double* m, uint32_t* id, double* r, double* v) {
+    History& h = *static_cast<History*>(hp);
+    if (n_bodies) *n_bodies = static_cast<uint32_t>(h.pl_alive);
+    for (size_t k = 0; k < h.nmax; k++) {
+        if (m) m[k] = h.pm[k];
+        if (id) id[k] = h.pid[k];
+        if (r) store3(r, k, h.pr[k]);
+        if (v) store3(v, k, h.pv[k]);
+    }
+}
+
+// Next planet block of at most tbsize steps: its start time, step, length,
+// bodies (GM, ids), position log (n_steps x (n_bodies-1) x 3) and h0 log.
+int orc_plhist_block(void* hp, uint32_t tbsize, double* t_start, double* dt, uint32_t* n_steps, uint32_t* n_bodies,
+                     double* m, uint32_t* id, double* r_log, double* h0_log) {
+    return static_cast<History*>(hp)->block(tbsize, t_start, dt, n_steps, n_bodies, m, id, r_log, h0_log);
 }
 
 }  // extern "C"
