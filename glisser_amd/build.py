@@ -26,7 +26,7 @@ HIP_SOURCES = ["wh_kernels.hip", "compact.hip", "interp.hip", "batch.hip", "trac
 # Host-only C++ (planet producer, conversions): g++ with the reference CPU
 # build's arithmetic -- no FP contraction, no fast-math.
 HOST_SOURCES = ["host/convert.cpp", "host/planets.cpp", "host/host_api.cpp", "host/executor.cpp", "host/formats.cpp",
-                "host/formats_api.cpp"]
+                "host/formats_api.cpp", "host/history.cpp", "host/history_api.cpp"]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-fno-fast-math"]
 
 # Packed-FP32 instructions stay off: plain float code must not be fused into

@@ -177,6 +177,11 @@ SIGNATURES = {
                                  _u64p]),
     "glx_track_write_records": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.POINTER(StateC), C.c_int, C.c_int,
                                           C.c_void_p, C.c_uint64]),
+    "glx_plhist_write": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_uint32, _dp, _dp, _dp, _u32p]),
+    "glx_plhist_open": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "glx_plhist_close": (C.c_int, [C.c_void_p]),
+    "glx_plhist_state": (C.c_int, [C.c_void_p, _u32p, _dp, _u32p, _dp, _dp]),
+    "glx_plhist_block": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _dp, _u32p, _u32p, _dp, _u32p, _dp, _dp]),
     "glx_planets_create": (C.c_int, [C.c_uint32, _dp, _dp, _dp, _u32p, C.c_double, C.c_double, C.c_double,
                                      C.POINTER(C.c_void_p)]),
     "glx_planets_destroy": (C.c_int, [C.c_void_p]),
@@ -739,6 +744,62 @@ def write_track(path, time: float, st: dict, to_elements: bool = True, barycentr
     c, _keep = _state_c(st)
     _check_host(load().glx_track_write(str(path).encode(), int(append), time, C.byref(c), int(to_elements),
                                        int(barycentric)))
+
+
+def write_plhist(path, time: float, m, r, v, ids=None, append: bool = False) -> None:
+    """Start (or, with append, extend) a planet history file with one record
+    of heliocentric bodies (entry 0 the Sun)."""
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
+    _check_host(load().glx_plhist_write(str(path).encode(), int(append), time, len(m), _ptr(m, _dp), _ptr(r, _dp),
+                                        _ptr(v, _dp), _ptr(ids, _u32p)))
+
+
+class PlanetHistory:
+    """A planet history file read block by block (element-linear
+    interpolation, blocks aligned to its records)."""
+
+    def __init__(self, path, t0: float, dt: float, table_size: int = 16):
+        self.lib = load()
+        self.table_size = table_size
+        h = C.c_void_p()
+        _check_host(self.lib.glx_plhist_open(str(path).encode(), t0, dt, table_size, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.glx_plhist_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def state(self) -> dict:
+        nb = np.zeros(1, np.uint32)
+        m, ids = np.empty(self.table_size), np.empty(self.table_size, np.uint32)
+        r, v = np.empty((self.table_size, 3)), np.empty((self.table_size, 3))
+        _check_host(self.lib.glx_plhist_state(self.h, _ptr(nb, _u32p), _ptr(m, _dp), _ptr(ids, _u32p), _ptr(r, _dp),
+                                              _ptr(v, _dp)))
+        return {"n_bodies": int(nb[0]), "m": m, "id": ids, "r": r, "v": v}
+
+    def block(self, tbsize: int) -> dict:
+        t, dt = np.zeros(1), np.zeros(1)
+        ns, nb = np.zeros(1, np.uint32), np.zeros(1, np.uint32)
+        m, ids = np.empty(self.table_size), np.empty(self.table_size, np.uint32)
+        rl = np.empty(tbsize * max(self.table_size - 1, 1) * 3)
+        h0 = np.empty((tbsize, 3))
+        _check_host(self.lib.glx_plhist_block(self.h, tbsize, _ptr(t, _dp), _ptr(dt, _dp), _ptr(ns, _u32p),
+                                              _ptr(nb, _u32p), _ptr(m, _dp), _ptr(ids, _u32p), _ptr(rl, _dp),
+                                              _ptr(h0, _dp)))
+        n, b = int(ns[0]), int(nb[0])
+        return {"t": float(t[0]), "dt": float(dt[0]), "n_steps": n, "n_bodies": b, "m": m[:b].copy(),
+                "id": ids[:b].copy(), "r_log": rl[: n * (b - 1) * 3].reshape(n, b - 1, 3).copy(),
+                "h0_log": h0[:n].copy()}
 
 
 def load_h0(m, r_log) -> np.ndarray:

@@ -417,6 +417,49 @@ int glx_track_write_records(const char* path, int append, double time, const glx
                             int barycentric, const void* records, uint64_t n);
 
 /* ---------------------------------------------------------------------
+ * Planet history files (README.md:154-171): the reference's "planetary
+ * history" of float32 heliocentric elements per record, written by
+ * save_swift_plhist (src/data.cpp:817-873) and read back by its
+ * Interpolator (src/interp.cpp), which moves a, e, i, Omega, omega
+ * linearly between records and the mean anomaly at a corrected constant
+ * rate.  Blocks never cross a record: each interval of length T holds
+ * max(1, round(T / dt)) steps of equal size.  Rows equal the reference CPU
+ * build's bit for bit.
+ * ------------------------------------------------------------------- */
+
+typedef struct glx_plhist glx_plhist;
+
+/* Start (append = 0: header with the Sun's GM first) or extend a history
+ * file with one record of n_bodies heliocentric bodies (entry 0 the Sun;
+ * id NULL means 0..n_bodies-1): begin_swift_plhist + save_swift_plhist. */
+int glx_plhist_write(const char* path, int append, double time, uint32_t n_bodies, const double* m, const double* r,
+                     const double* v, const uint32_t* id);
+
+/* Open a history for a run starting at t0 with Time-Step dt and a planet
+ * table of table_size slots (Planet-History-Max-Planet-Count, the Sun
+ * included): the Interpolator constructor and the history set-up of
+ * Executor::init (src/interp.cpp:13-91, src/executor.cu:93-130).
+ * GLX_EINVAL with the reference's message if the history starts after t0 or
+ * ends before it. */
+int glx_plhist_open(const char* path, double t0, double dt, uint32_t table_size, glx_plhist** out);
+int glx_plhist_close(glx_plhist* h);
+
+/* The planet table (table_size slots, the first *n_bodies in use): GM, ids
+ * and heliocentric r, v of the last interpolated time (at open: t0).  Any
+ * output may be NULL. */
+int glx_plhist_state(glx_plhist* h, uint32_t* n_bodies, double* m, uint32_t* id, double* r, double* v);
+
+/* The next planet block of at most tbsize steps: *t_start, step *dt,
+ * *n_steps, bodies (*n_bodies GM values and ids, Sun first), r_log
+ * (n_steps x (n_bodies-1) x 3, row s at t_start + (s+1) dt; room for
+ * tbsize x (table_size-1) x 3) and h0_log (n_steps x 3, may be NULL).
+ * Executor::update_planets with Interpolator::fill and load_h0
+ * (src/executor.cu:235-292, src/interp.cpp:113-165, src/wh.cpp:993-1025).
+ * GLX_EINVAL at the end of the file ("EOS reached for lookup file."). */
+int glx_plhist_block(glx_plhist* h, uint32_t tbsize, double* t_start, double* dt, uint32_t* n_steps,
+                     uint32_t* n_bodies, double* m, uint32_t* id, double* r_log, double* h0_log);
+
+/* ---------------------------------------------------------------------
  * Host-side producer of the kernel's planet input and orbital conversions.
  * Built without FP contraction so it rounds like the reference CPU build.
  * ------------------------------------------------------------------- */

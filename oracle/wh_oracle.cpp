@@ -23,6 +23,10 @@
 //   to_elements/from_elements  : round trip, circular/planar special cases
 //   hermite_log                : exact for cubics, node reproduction,
 //                                fourth-order convergence on Kepler orbits
+//   planet history (plhist_*)  : writer bytes and interpolated blocks equal
+//                                the reference build's and its golden blocks;
+//                                intervals end on the records' own elements;
+//                                Keplerian records give Kepler motion
 //
 // Error behaviour: where the reference throws std::runtime_error (unbound
 // orbit, unconverged Kepler solve; src/wh.cpp:894-908,942-945) the oracle

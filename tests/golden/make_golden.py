@@ -8,9 +8,10 @@ root:
 
     make -C oracle ref && python tests/golden/make_golden.py [part ...]
 
-The fixtures are small .npz files (inputs and expected outputs only) and,
-under formats/, state, configuration and track files written by the
-reference's own writers.
+The fixtures are small .npz files (inputs and expected outputs only),
+under formats/ state, configuration and track files written by the
+reference's own writers, and under plhist/ a synthetic planet history with
+the blocks the reference's Interpolator makes of it.
 """
 from __future__ import annotations
 
@@ -126,11 +127,40 @@ def formats(ref):
     (d / "config.out").write_text(ref.config_normalize(CONFIG))
 
 
+def plhist(ref):
+    """A planet history (au, years; planets dropping out and appearing) and
+    the blocks the reference's own Interpolator makes of it, driven as its
+    executor drives it."""
+    from tests.helpers import GM_SUN_YR, sample_history, write_plhist_bytes
+    d = OUT / "plhist"
+    d.mkdir(exist_ok=True)
+    write_plhist_bytes(d / "hist.bin", GM_SUN_YR, sample_history(21, n_records=8, spacing=1.5, n_planets=4,
+                                                                  dropout=True, newcomer=True))
+    t0, dt, tb = 0.75, 0.04, 16
+    h = ref.plhist(d / "hist.bin", t0, dt)
+    out = {"t0": t0, "dt": dt, "tbsize": tb}
+    blocks = []
+    while True:
+        try:
+            blocks.append(h.block(tb))
+        except Exception:
+            break
+    out["n_blocks"] = len(blocks)
+    out["t"] = np.array([b["t"] for b in blocks])
+    out["dt_block"] = np.array([b["dt"] for b in blocks])
+    out["n_steps"] = np.array([b["n_steps"] for b in blocks])
+    for k, b in enumerate(blocks):
+        out[f"r_log_{k}"] = b["r_log"]
+        out[f"h0_log_{k}"] = b["h0_log"]
+        out[f"id_{k}"] = b["id"]
+    np.savez_compressed(d / "blocks.npz", **out)
+
+
 if __name__ == "__main__":
     ref = Oracle(reference=True)
     parts = sys.argv[1:] or ["planets", "particle_block", "long_run", "encounters", "partition", "elements",
-                             "formats"]
+                             "formats", "plhist"]
     for name in parts:
         globals()[name](ref)
-    for f in sorted(OUT.glob("*.npz")) + sorted((OUT / "formats").glob("*")):
+    for f in sorted(OUT.glob("*.npz")) + sorted((OUT / "formats").glob("*")) + sorted((OUT / "plhist").glob("*")):
         print(f.name, f.stat().st_size)

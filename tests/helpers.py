@@ -104,3 +104,53 @@ def compare_flags(cpu_flags, gpu_flags, cpu_ids, gpu_ids):
     low = ids[(fc & 0xFF) != (fg & 0xFF)]
     high = ids[(fc >> 8) != (fg >> 8)]
     return low, high
+
+
+# au, years: GM of the Sun = 4 pi^2, the units the reference's mean-motion
+# estimate of a history interval assumes (src/interp.cpp:271-272)
+GM_SUN_YR = 4 * np.pi ** 2
+
+
+def write_plhist_bytes(path, sun_gm, records):
+    """A planet history file in the reference's layout (README.md:154-171):
+    records = [(time, [(id, gm, a, e, i, Omega, omega, M), ...]), ...]."""
+    import struct
+    with open(path, "wb") as f:
+        f.write(struct.pack("<d", sun_gm) + b"\0" * 24)
+        for t, pls in records:
+            f.write(struct.pack("<dI", t, len(pls)) + b"\0" * 20)
+            for p in pls:
+                f.write(struct.pack("<I7f", *p))
+
+
+def sample_history(seed, n_records=30, spacing=1.0, n_planets=4, dropout=False, newcomer=False, e_mode=None,
+                   sun_gm=GM_SUN_YR, drift=True):
+    """Synthetic history records (au, years): planets whose mean anomaly
+    advances at the two-body rate (plus a small drift unless drift=False),
+    every eccentricity regime of the reference's Kepler solver (e < 0.18,
+    < 0.8, >= 0.8), optionally a planet absent for a few records (dropout)
+    and one that appears part way (newcomer)."""
+    rng = np.random.default_rng(seed)
+    base = []
+    for j in range(n_planets):
+        regime = e_mode if e_mode is not None else j % 3
+        e = (rng.uniform(0, 0.17), rng.uniform(0.18, 0.79), rng.uniform(0.8, 0.95))[regime]
+        base.append((100 + 7 * j, sun_gm * 10 ** rng.uniform(-5, -3), rng.uniform(1, 40), e, rng.uniform(0, 0.5),
+                     rng.uniform(-np.pi, np.pi), rng.uniform(-np.pi, np.pi), rng.uniform(-np.pi, np.pi)))
+    wrap = lambda x: (x + np.pi) % (2 * np.pi) - np.pi  # noqa: E731
+    records = []
+    for k in range(n_records):
+        pls = []
+        for j, (pid, gm, a, e, inc, node, peri, m0) in enumerate(base):
+            if dropout and j == 0 and 10 <= k < 14:
+                continue
+            n = np.sqrt((sun_gm + gm) / a ** 3)
+            if drift:
+                pls.append((pid, gm, a * (1 + 1e-4 * np.sin(0.3 * k)), e * (1 + 1e-3 * np.cos(0.2 * k)), inc,
+                            wrap(node + 1e-3 * k), wrap(peri - 2e-3 * k), wrap(m0 + (n + 1e-6 * j) * spacing * k)))
+            else:
+                pls.append((pid, gm, a, e, inc, node, peri, wrap(m0 + n * spacing * k)))
+        if newcomer and k >= 8:
+            pls.append((999, sun_gm * 1e-4, 12.0, 0.3, 0.1, 1.0, 2.0, wrap(0.5 + 0.2 * k)))
+        records.append((k * spacing, pls))
+    return records
