@@ -15,6 +15,7 @@
 #include "../errors.h"
 #include "glisser_amd.h"
 #include "glisser_amd_executor.hpp"
+#include "history.h"
 #include "planets.h"
 
 namespace glx {
@@ -35,17 +36,30 @@ double ms_since(Clock::time_point t0) {
 
 struct Executor::Impl {
     ExecutorConfig cfg;
-    host::PlanetIntegrator planets;
+    // planets: integrated on the host, or interpolated from a history file
+    std::unique_ptr<host::PlanetIntegrator> planets;
+    std::unique_ptr<host::PlanetHistory> history;
     glx_ctx* ctx = nullptr;
-    uint32_t nb = 0;
+    uint32_t nb_max = 0;
     uint64_t n_total = 0, n_alive = 0;
     double t = 0.0;
     int buf = 0;
-    // planet row at time t (positions of bodies 1..nb-1, then h0) and the
-    // encounter radii of the block that starts at t
-    std::vector<double> row_r, row_h0, rh_next;
-    // next block, produced while the kernel runs
-    std::vector<double> log_r, log_v, log_h0;
+    // one planet block: start time, step, length, bodies and their radii,
+    // position / velocity / h0 logs
+    struct Block {
+        double t0 = 0, dt = 0;
+        uint32_t n_steps = 0, nb = 0;
+        std::vector<double> m, rh, log_r, log_v, log_h0;
+        std::vector<uint32_t> id;
+    };
+    // next block, produced while the kernel runs the current one
+    Block next;
+    // the planets at time t (positions of bodies 1..nb-1, then h0), with the
+    // bodies and radii they were stepped with: the input of glx_accel_range
+    // for particles handed back at t
+    uint32_t row_nb = 0;
+    std::vector<double> row_r, row_h0, row_m, row_rh;
+    std::vector<uint32_t> row_id;
     std::vector<EncounterRecord> queue;
     // time of the flagging step by particle id: a flat table when the ids
     // are dense (max id < 4 n), else a hash map
@@ -70,41 +84,90 @@ struct Executor::Impl {
         return it == deathtime_map.end() ? std::numeric_limits<double>::quiet_NaN() : it->second;
     }
     double planet_ms = 0.0;
+    std::string history_end;  // set once a planet history has no block left
 
-    Impl(const ExecutorConfig& c, host::PlanetIntegrator&& p) : cfg(c), planets(std::move(p)) {}
+    explicit Impl(const ExecutorConfig& c) : cfg(c) {}
 
     ~Impl() {
         if (ctx) glx_destroy(ctx);
     }
 
-    // Planet block for [t, t + tbsize dt] into log_*, with the radii of its start.
-    void produce_block() {
-        const auto t0 = Clock::now();
-        planets.recalculate_rh();
-        rh_next = planets.rh();
-        const size_t np = nb - 1;
-        log_r.resize((size_t)cfg.tbsize * np * 3);
-        log_v.resize((size_t)cfg.tbsize * np * 3);
-        log_h0.resize((size_t)cfg.tbsize * 3);
-        planets.integrate_block(cfg.tbsize, cfg.dt, log_r.data(), log_v.data(), log_h0.data());
-        planet_ms = ms_since(t0);
+    // The block that starts at t_start into `next`, with the radii of its
+    // start (recalculate_rh before the planets move, as update_planets does,
+    // src/executor.cu:226-299).
+    void produce_block(double t_start) {
+        const auto c0 = Clock::now();
+        Block& b = next;
+        if (history) {
+            const host::PlanetHistory& h = *history;
+            std::vector<double> rh_all(h.table_size());
+            host::encounter_radii(h.table_size(), h.m().data(), h.r().data(), h.v().data(), cfg.cull_radius,
+                                  cfg.rh_factor, rh_all.data());
+            host::HistoryBlock hb;
+            try {
+                history->next_block(cfg.tbsize, &hb);
+            } catch (const host::HistoryEnd& e) {
+                // the run ends with the history; the block before still runs
+                b.n_steps = 0;
+                history_end = e.what();
+                planet_ms = ms_since(c0);
+                return;
+            }
+            b.t0 = hb.t_start;
+            b.dt = hb.dt;
+            b.n_steps = hb.n_steps;
+            b.nb = hb.n_bodies;
+            b.m = std::move(hb.m);
+            b.id = std::move(hb.id);
+            b.rh.assign(rh_all.begin(), rh_all.begin() + b.nb);
+            b.log_r = std::move(hb.r_log);
+            b.log_h0 = std::move(hb.h0_log);
+        } else {
+            planets->recalculate_rh();
+            b.t0 = t_start;
+            b.dt = cfg.dt;
+            b.n_steps = cfg.tbsize;
+            b.nb = static_cast<uint32_t>(planets->n());
+            b.m = planets->m();
+            b.id = planets->id();
+            b.rh = planets->rh();
+            const size_t np = b.nb - 1;
+            b.log_r.resize((size_t)cfg.tbsize * np * 3);
+            b.log_v.resize((size_t)cfg.tbsize * np * 3);
+            b.log_h0.resize((size_t)cfg.tbsize * 3);
+            planets->integrate_block(cfg.tbsize, cfg.dt, b.log_r.data(), b.log_v.data(), b.log_h0.data());
+        }
+        planet_ms = ms_since(c0);
     }
 
     void upload_block() {
-        check(glx_upload_planet_block(ctx, buf, nb, planets.m().data(), planets.id().data(), rh_next.data(),
-                                      log_r.data(), log_h0.data(), cfg.tbsize),
+        const Block& b = next;
+        if (b.n_steps == 0) return;  // past the end of a planet history
+        check(glx_upload_planet_block(ctx, buf, b.nb, b.m.data(), b.id.data(), b.rh.data(), b.log_r.data(),
+                                      b.log_h0.data(), b.n_steps),
               "glx_upload_planet_block");
+    }
+
+    // the planets at the end of block b become the row at time t
+    void take_row(const Block& b) {
+        const size_t np = b.nb - 1;
+        row_nb = b.nb;
+        row_m = b.m;
+        row_id = b.id;
+        row_rh = b.rh;
+        row_r.assign(b.log_r.begin() + (size_t)(b.n_steps - 1) * np * 3, b.log_r.begin() + (size_t)b.n_steps * np * 3);
+        row_h0.assign(b.log_h0.begin() + (size_t)(b.n_steps - 1) * 3, b.log_h0.begin() + (size_t)b.n_steps * 3);
     }
 
     // Records particles [begin, end) flagged during the block that started at
     // t_block; in resolve mode the encounter class [begin, begin + n_enc)
     // also goes to the queue with its rollback state.
-    void record_flagged(uint64_t begin, uint64_t end, uint64_t n_enc, double t_block) {
+    void record_flagged(uint64_t begin, uint64_t end, uint64_t n_enc, double t_block, double dt_block) {
         const uint64_t n = end - begin;
         if (n == 0) return;
         std::vector<glx_encounter> rec(n);
         check(glx_pack_encounters(ctx, begin, n, t_block, rec.data()), "glx_pack_encounters");
-        for (uint64_t k = 0; k < n; k++) set_deathtime(rec[k].id, t_block + (double)rec[k].step * cfg.dt);
+        for (uint64_t k = 0; k < n; k++) set_deathtime(rec[k].id, t_block + (double)rec[k].step * dt_block);
         queue.insert(queue.end(), rec.begin(), rec.begin() + n_enc);
     }
 
@@ -123,27 +186,69 @@ struct Executor::Impl {
 
 Executor::Executor(const ExecutorConfig& cfg, uint32_t n_bodies, const double* m, const double* r, const double* v,
                    const uint32_t* planet_ids, uint64_t n, const double* pr, const double* pv, const uint32_t* ids) {
-    if (n_bodies < 1 || n_bodies > GLX_MAX_BODIES) throw std::runtime_error("Executor: n_bodies outside [1, 16]");
-    if (!m || !r || !v || (n && (!pr || !pv || !ids))) throw std::runtime_error("Executor: null argument");
-    if (cfg.tbsize < 1 || !(cfg.dt > 0.0)) throw std::runtime_error("Executor: tbsize and dt must be positive");
-    std::vector<double> mm(m, m + n_bodies);
-    std::vector<host::Vec3> rr(n_bodies), vv(n_bodies);
-    std::vector<uint32_t> pid(n_bodies);
-    for (uint32_t i = 0; i < n_bodies; i++) {
-        rr[i] = host::Vec3::load(r, i);
-        vv[i] = host::Vec3::load(v, i);
-        pid[i] = planet_ids ? planet_ids[i] : i;
+    const bool from_history = !cfg.planet_history.empty();
+    if (!from_history) {
+        if (n_bodies < 1 || n_bodies > GLX_MAX_BODIES) throw std::runtime_error("Executor: n_bodies outside [1, 16]");
+        if (!m || !r || !v) throw std::runtime_error("Executor: null planet argument");
+    } else if (cfg.history_table_size < 1 || cfg.history_table_size > GLX_MAX_BODIES) {
+        throw std::runtime_error("Executor: history_table_size outside [1, 16]");
     }
-    p_ = std::make_unique<Impl>(cfg, host::PlanetIntegrator(std::move(mm), std::move(rr), std::move(vv),
-                                                            std::move(pid), cfg.cull_radius, cfg.rh_factor,
-                                                            cfg.outer_radius));
+    if (n && (!pr || !pv || !ids)) throw std::runtime_error("Executor: null particle argument");
+    if (cfg.tbsize < 1 || !(cfg.dt > 0.0)) throw std::runtime_error("Executor: tbsize and dt must be positive");
+    p_ = std::make_unique<Impl>(cfg);
     Impl& s = *p_;
-    s.nb = n_bodies;
+    // planets at t0: bodies, positions, velocities, radii, h0
+    std::vector<double> m0, rh0, row_r, h0(3);
+    std::vector<uint32_t> id0;
+    if (from_history) {
+        // Executor::init with Read-Planet-History (src/executor.cu:93-133)
+        s.history = std::make_unique<host::PlanetHistory>(cfg.planet_history, cfg.t0, cfg.dt, cfg.history_table_size);
+        const host::PlanetHistory& h = *s.history;
+        const uint32_t nb = h.n_bodies();
+        std::vector<double> rh_all(h.table_size());
+        host::encounter_radii(h.table_size(), h.m().data(), h.r().data(), h.v().data(), cfg.cull_radius, cfg.rh_factor,
+                              rh_all.data());
+        m0.assign(h.m().begin(), h.m().begin() + nb);
+        id0.assign(h.id().begin(), h.id().begin() + nb);
+        rh0.assign(rh_all.begin(), rh_all.begin() + nb);
+        row_r.resize((size_t)(nb - 1) * 3);
+        for (uint32_t i = 1; i < nb; i++) h.r()[i].store(row_r.data(), i - 1);
+        if (nb > 1) host::h0_from_log(nb, m0.data(), row_r.data(), 1, h0.data());
+        s.nb_max = cfg.history_table_size;
+    } else {
+        std::vector<double> mm(m, m + n_bodies);
+        std::vector<host::Vec3> rr(n_bodies), vv(n_bodies);
+        std::vector<uint32_t> pid(n_bodies);
+        for (uint32_t i = 0; i < n_bodies; i++) {
+            rr[i] = host::Vec3::load(r, i);
+            vv[i] = host::Vec3::load(v, i);
+            pid[i] = planet_ids ? planet_ids[i] : i;
+        }
+        s.planets = std::make_unique<host::PlanetIntegrator>(std::move(mm), std::move(rr), std::move(vv),
+                                                             std::move(pid), cfg.cull_radius, cfg.rh_factor,
+                                                             cfg.outer_radius);
+        // the WHIntegrator constructor (src/wh.cpp:498-543)
+        const host::PlanetIntegrator& pl = *s.planets;
+        m0 = pl.m();
+        id0 = pl.id();
+        rh0 = pl.rh();
+        row_r.resize((size_t)(n_bodies - 1) * 3);
+        for (uint32_t i = 1; i < n_bodies; i++) pl.r()[i].store(row_r.data(), i - 1);
+        const host::Vec3 hv = pl.h0_initial();
+        h0 = {hv.x, hv.y, hv.z};
+        s.nb_max = n_bodies;
+    }
+    s.row_nb = static_cast<uint32_t>(m0.size());
+    s.row_m = m0;
+    s.row_id = id0;
+    s.row_rh = rh0;
+    s.row_r = row_r;
+    s.row_h0 = h0;
     s.n_total = n;
     s.t = cfg.t0;
     glx_params gp{};
     gp.n_particles = n;
-    gp.n_bodies_max = n_bodies;
+    gp.n_bodies_max = s.nb_max;
     gp.tbsize_max = cfg.tbsize;
     gp.device = cfg.device;
     gp.keep_rollback = cfg.resolve_encounters ? 1 : 0;
@@ -160,49 +265,45 @@ Executor::Executor(const ExecutorConfig& cfg, uint32_t n_bodies, const double* m
         check(glx_upload_particles(s.ctx, 0, n, pr, pv, a.data(), flags.data(), ids, dti.data()),
               "glx_upload_particles");
     }
-    // planets at t0: positions, h0 and radii (the WHIntegrator constructor,
-    // src/wh.cpp:498-543)
-    s.row_r.resize((size_t)(n_bodies - 1) * 3);
-    for (uint32_t i = 1; i < n_bodies; i++) s.planets.r()[i].store(s.row_r.data(), i - 1);
-    const host::Vec3 h0 = s.planets.h0_initial();
-    s.row_h0 = {h0.x, h0.y, h0.z};
     check(glx_set_counts(s.ctx, n, n), "glx_set_counts");
     if (n) {
-        check(glx_init_accel(s.ctx, n_bodies, s.planets.m().data(), s.planets.id().data(), s.planets.rh().data(),
-                             s.row_r.data(), s.row_h0.data(), cfg.outer_radius),
+        check(glx_init_accel(s.ctx, s.row_nb, s.row_m.data(), s.row_id.data(), s.row_rh.data(), s.row_r.data(),
+                             s.row_h0.data(), cfg.outer_radius),
               "glx_init_accel");
         if (cfg.resolve_encounters) check(glx_snapshot_rollback(s.ctx), "glx_snapshot_rollback");
         const uint64_t ne = s.compact(n);
-        s.record_flagged(s.n_alive, n, cfg.resolve_encounters ? ne : 0, cfg.t0);
+        s.record_flagged(s.n_alive, n, cfg.resolve_encounters ? ne : 0, cfg.t0, cfg.dt);
     }
-    s.produce_block();
+    s.produce_block(cfg.t0);
     s.upload_block();
+    if (s.next.n_steps) s.t = s.next.t0;
 }
 
 Executor::~Executor() = default;
 
 void Executor::step() {
     Impl& s = *p_;
+    if (s.next.n_steps == 0) throw std::runtime_error("planet history ended at t = " + std::to_string(s.t) + ": " +
+                                                      s.history_end);
     const uint64_t prev = s.n_alive;
-    const double t_block = s.t;
+    const double t_block = s.next.t0, dt_block = s.next.dt;
+    const uint32_t n_steps = s.next.n_steps;
     check(glx_set_counts(s.ctx, prev, s.n_total), "glx_set_counts");
     if (prev) {
         if (s.cfg.resolve_encounters) check(glx_snapshot_rollback(s.ctx), "glx_snapshot_rollback");
-        check(glx_run_block(s.ctx, s.buf, s.cfg.dt, s.cfg.outer_radius, s.cfg.form), "glx_run_block");
+        check(glx_run_block(s.ctx, s.buf, dt_block, s.cfg.outer_radius, s.cfg.form), "glx_run_block");
     }
-    // the planet row at the end of this block, then the next block (the
-    // host work overlaps the kernel)
-    const size_t np = s.nb - 1;
-    std::memcpy(s.row_r.data(), s.log_r.data() + (size_t)(s.cfg.tbsize - 1) * np * 3, np * 3 * sizeof(double));
-    std::memcpy(s.row_h0.data(), s.log_h0.data() + (size_t)(s.cfg.tbsize - 1) * 3, 3 * sizeof(double));
-    s.produce_block();
+    // the planets at the end of this block, then the next block (the host
+    // work overlaps the kernel)
+    s.take_row(s.next);
+    s.produce_block(t_block + (double)n_steps * dt_block);
     if (prev) {
         const uint64_t ne = s.compact(prev);
-        s.record_flagged(s.n_alive, prev, s.cfg.resolve_encounters ? ne : 0, t_block);
+        s.record_flagged(s.n_alive, prev, s.cfg.resolve_encounters ? ne : 0, t_block, dt_block);
     }
     s.buf ^= 1;
     s.upload_block();
-    s.t = t_block + (double)s.cfg.tbsize * s.cfg.dt;
+    s.t = s.next.n_steps ? s.next.t0 : t_block + (double)n_steps * dt_block;
 }
 
 void Executor::run(uint32_t n_blocks) {
@@ -250,7 +351,7 @@ void Executor::return_particles(uint64_t n, const uint32_t* ids, const double* r
     uint64_t na = 0, ne = 0;
     check(glx_partition(s.ctx, GLX_PART_THREE_WAY, &na, &ne), "glx_partition");
     if (na != begin + n) throw std::runtime_error("return_particles: compaction count mismatch");
-    check(glx_accel_range(s.ctx, begin, n, s.nb, s.planets.m().data(), s.planets.id().data(), s.rh_next.data(),
+    check(glx_accel_range(s.ctx, begin, n, s.row_nb, s.row_m.data(), s.row_id.data(), s.row_rh.data(),
                           s.row_r.data(), s.row_h0.data(), s.cfg.outer_radius),
           "glx_accel_range");
     // clear a fresh encounter flag: the next step detects a lasting one
@@ -288,6 +389,9 @@ ParticleState Executor::download() const {
     return out;
 }
 
+uint32_t Executor::n_bodies() const { return p_->row_nb; }
+std::vector<double> Executor::planet_masses() const { return p_->row_m; }
+std::vector<uint32_t> Executor::planet_ids() const { return p_->row_id; }
 std::vector<double> Executor::planet_positions() const { return p_->row_r; }
 
 void Executor::last_times(double* kernel_ms, double* partition_ms, double* planet_ms) const {
@@ -340,6 +444,8 @@ int glx_exec_create(const glx_exec_params* p, uint32_t n_bodies, const double* m
         c.resolve_encounters = p->resolve_encounters != 0;
         c.form = p->form;
         c.device = p->device;
+        if (p->planet_history) c.planet_history = p->planet_history;
+        c.history_table_size = p->history_table_size ? p->history_table_size : 16;
         auto e = std::make_unique<glx_exec>();
         e->ex = std::make_unique<glx::Executor>(c, n_bodies, m, r, v, planet_id, n, pr, pv, ids);
         *out = e.release();
@@ -412,6 +518,21 @@ int glx_exec_planets(glx_exec* e, double* r) {
     if (!e || !r) return glx::set_error(GLX_EINVAL, "null argument");
     const auto p = e->ex->planet_positions();
     std::memcpy(r, p.data(), p.size() * sizeof(double));
+    return GLX_OK;
+}
+
+int glx_exec_bodies(glx_exec* e, uint32_t* n_bodies, double* m, uint32_t* id) {
+    if (!e) return glx::set_error(GLX_EINVAL, "null executor");
+    const uint32_t nb = e->ex->n_bodies();
+    if (n_bodies) *n_bodies = nb;
+    if (m) {
+        const auto v = e->ex->planet_masses();
+        std::memcpy(m, v.data(), v.size() * sizeof(double));
+    }
+    if (id) {
+        const auto v = e->ex->planet_ids();
+        std::memcpy(id, v.data(), v.size() * sizeof(uint32_t));
+    }
     return GLX_OK;
 }
 

@@ -83,7 +83,10 @@ size_t PlanetHistory::slot(uint32_t id) {
 
 void PlanetHistory::read_record_header() {
     t_end_ = read_le<double>(in_);
-    if (!in_) throw std::runtime_error("EOS reached for lookup file.");
+    if (!in_) {
+        ended_ = true;
+        throw HistoryEnd();
+    }
     n_end_ = read_le<uint32_t>(in_) + 1;
     in_.seekg(20, std::ios_base::cur);
 }
@@ -201,6 +204,7 @@ void PlanetHistory::state_at(const Track& k, double rel, Vec3* r, Vec3* v) const
 // the time advance of src/executor.cu:700.
 void PlanetHistory::next_block(uint32_t tbsize, HistoryBlock* out) {
     if (tbsize < 1) throw std::runtime_error("planet history: tbsize must be positive");
+    if (ended_) throw HistoryEnd();
     if (cur_ts_ == n_ts_) {
         const double diff = rel_t_ - (t_end_ - t_begin_);
         if (!(std::abs(diff) < 1e-8))

@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <fstream>
 #include <iosfwd>
+#include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
@@ -36,6 +37,11 @@ namespace host {
 // the file header (the Sun's GM) is written before it.
 void write_history_record(std::ostream& out, bool first, double time, uint32_t n, const double* m, const Vec3* r,
                           const Vec3* v, const uint32_t* id);
+
+// Thrown when a block is asked for past the last record of the file.
+struct HistoryEnd : std::runtime_error {
+    HistoryEnd() : std::runtime_error("EOS reached for lookup file.") {}
+};
 
 // One planet block from a history: start time, step, number of steps,
 // bodies (entry 0 the Sun), position log (n_steps x (n_bodies-1) x 3,
@@ -58,7 +64,8 @@ public:
     // after t0, ends before it, or names more planets than the table holds.
     PlanetHistory(const std::string& path, double t0, double dt, uint32_t table_size);
 
-    // The next block, at most tbsize steps; throws at the end of the file.
+    // The next block, at most tbsize steps; throws HistoryEnd at the end of
+    // the file.
     void next_block(uint32_t tbsize, HistoryBlock* out);
 
     // Planet table (table_size slots; the first n_bodies() in use): GM, ids,
@@ -103,6 +110,7 @@ private:
     size_t n_ts_ = 0, cur_ts_ = 0;
     double eff_dt_ = 0, rel_t_ = 0;
     double t_ = 0;
+    bool ended_ = false;  // the last record has been passed
 };
 
 }  // namespace host

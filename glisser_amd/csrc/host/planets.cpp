@@ -100,17 +100,21 @@ PlanetIntegrator::PlanetIntegrator(std::vector<double> m, std::vector<Vec3> r, s
     h0_init_ = accelerations();
 }
 
-void PlanetIntegrator::recalculate_rh() {
-    const size_t n = m_.size();
-    if (std::abs(rh_factor_) > 1e-6) {
-        rh_[0] = cull_;
+void encounter_radii(size_t n, const double* m, const Vec3* r, const Vec3* v, double cull_radius, double rh_factor,
+                     double* rh) {
+    if (std::abs(rh_factor) > 1e-6) {
+        rh[0] = cull_radius;
         for (size_t i = 1; i < n; i++) {
-            const Elements el = to_elements(m_[0] + m_[i], r_[i], v_[i]);
-            rh_[i] = rh_factor_ * el.a * (1 - el.e) * std::pow(m_[i] / (3 * m_[0]), 1. / 3);
+            const Elements el = to_elements(m[0] + m[i], r[i], v[i]);
+            rh[i] = rh_factor * el.a * (1 - el.e) * std::pow(m[i] / (3 * m[0]), 1. / 3);
         }
     } else {
-        for (size_t i = 0; i < n; i++) rh_[i] = cull_;
+        for (size_t i = 0; i < n; i++) rh[i] = cull_radius;
     }
+}
+
+void PlanetIntegrator::recalculate_rh() {
+    encounter_radii(m_.size(), m_.data(), r_.data(), v_.data(), cull_, rh_factor_, rh_.data());
 }
 
 Vec3 PlanetIntegrator::accelerations() {

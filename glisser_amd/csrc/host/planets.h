@@ -65,6 +65,13 @@ private:
     Vec3 h0_init_;
 };
 
+// Encounter radii of bodies 0..n-1 (src/wh.cpp:472-495): with
+// |rh_factor| > 1e-6 the Sun gets cull_radius and planet i
+// rh_factor a (1 - e) (m_i / 3 m_0)^(1/3) from its heliocentric osculating
+// elements; otherwise every body gets cull_radius.
+void encounter_radii(size_t n, const double* m, const Vec3* r, const Vec3* v, double cull_radius, double rh_factor,
+                     double* rh);
+
 // Indirect-term rows from a position log (src/wh.cpp:993-1025).
 void h0_from_log(uint32_t n_bodies, const double* m, const double* r_log, uint32_t n_steps, double* h0_log);
 

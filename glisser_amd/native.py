@@ -13,6 +13,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libglisser_amd.so"
 
 GLX_OK, GLX_EINVAL, GLX_EHIP, GLX_ENOMEM, GLX_ESTATE = 0, 1, 2, 3, 4
+GLX_MAX_BODIES = 16
 FORM_FP64, FORM_FAST, FORM_MIXED = 0, 1, 2
 FORMS = {"fp64": FORM_FP64, "fast": FORM_FAST, "mixed": FORM_MIXED}
 PART_UNFLAGGED, PART_THREE_WAY = 0, 1
@@ -102,6 +103,8 @@ class ExecParams(C.Structure):
         ("resolve_encounters", C.c_int32),
         ("form", C.c_int32),
         ("device", C.c_int32),
+        ("planet_history", C.c_char_p),
+        ("history_table_size", C.c_uint32),
     ]
 
 
@@ -166,6 +169,7 @@ SIGNATURES = {
     "glx_exec_return": (C.c_int, [C.c_void_p, C.c_uint64, _u32p, _dp, _dp]),
     "glx_exec_download": (C.c_int, [C.c_void_p, _dp, _dp, _u16p, _u32p, _dp]),
     "glx_exec_planets": (C.c_int, [C.c_void_p, _dp]),
+    "glx_exec_bodies": (C.c_int, [C.c_void_p, _u32p, _dp, _u32p]),
     "glx_exec_times": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "glx_config_normalize": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint64, _u64p]),
     "glx_state_save": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(StateC)]),
@@ -586,21 +590,30 @@ class Executor:
 
     def __init__(self, m, r, v, pr, pv, ids=None, planet_ids=None, t0: float = 0.0, dt: float = 180.0,
                  tbsize: int = 2048, outer_radius: float = 300.0, cull_radius: float = 0.5, rh_factor: float = 0.0,
-                 resolve_encounters: bool = False, form: str | int = "fp64", device: int = 0):
+                 resolve_encounters: bool = False, form: str | int = "fp64", device: int = 0,
+                 planet_history=None, history_table_size: int = 16):
+        """m, r, v: planets (Sun first); with planet_history (a history file)
+        they are not used and may be None."""
         self.lib = load()
-        m = np.ascontiguousarray(m, dtype=np.float64)
-        self.nb = len(m)
-        r = _vec3(r, self.nb)
-        v = _vec3(v, self.nb)
+        if planet_history is None:
+            m = np.ascontiguousarray(m, dtype=np.float64)
+            nb = len(m)
+            r = _vec3(r, nb)
+            v = _vec3(v, nb)
+        else:
+            m = r = v = None
+            nb = 0
         n = len(pr)
         pr = _vec3(pr, n)
         pv = _vec3(pv, n)
         ids = np.arange(n, dtype=np.uint32) if ids is None else np.ascontiguousarray(ids, dtype=np.uint32)
         pid = None if planet_ids is None else np.ascontiguousarray(planet_ids, dtype=np.uint32)
         f = FORMS[form] if isinstance(form, str) else int(form)
-        p = ExecParams(t0, dt, tbsize, outer_radius, cull_radius, rh_factor, int(resolve_encounters), f, device)
+        hist = None if planet_history is None else str(planet_history).encode()
+        p = ExecParams(t0, dt, tbsize, outer_radius, cull_radius, rh_factor, int(resolve_encounters), f, device, hist,
+                       history_table_size)
         h = C.c_void_p()
-        rc = self.lib.glx_exec_create(C.byref(p), self.nb, _ptr(m, _dp), _ptr(r, _dp), _ptr(v, _dp),
+        rc = self.lib.glx_exec_create(C.byref(p), nb, _ptr(m, _dp), _ptr(r, _dp), _ptr(v, _dp),
                                       _ptr(pid, _u32p), n, _ptr(pr, _dp), _ptr(pv, _dp), _ptr(ids, _u32p),
                                       C.byref(h))
         if rc != 0:
@@ -667,10 +680,21 @@ class Executor:
         out.update(self.status())
         return out
 
+    def bodies(self) -> dict:
+        """The bodies at the current time: n_bodies, GM values and ids."""
+        nb = C.c_uint32()
+        m, ids = np.empty(GLX_MAX_BODIES), np.empty(GLX_MAX_BODIES, np.uint32)
+        self._check(self.lib.glx_exec_bodies(self.h, C.byref(nb), _ptr(m, _dp), _ptr(ids, _u32p)))
+        return {"n_bodies": nb.value, "m": m[:nb.value].copy(), "id": ids[:nb.value].copy()}
+
+    @property
+    def nb(self) -> int:
+        return self.bodies()["n_bodies"]
+
     def planet_positions(self) -> np.ndarray:
-        out = np.empty((self.nb - 1, 3))
+        out = np.empty((GLX_MAX_BODIES - 1) * 3)
         self._check(self.lib.glx_exec_planets(self.h, _ptr(out, _dp)))
-        return out
+        return out[: (self.nb - 1) * 3].reshape(-1, 3).copy()
 
     def times(self) -> tuple[float, float, float]:
         k, q, p = C.c_double(), C.c_double(), C.c_double()

@@ -325,11 +325,17 @@ typedef struct glx_exec_params {
     int32_t resolve_encounters; /* Resolve-Encounters: hand encounter particles to the host queue */
     int32_t form;               /* glx_form */
     int32_t device;
+    const char* planet_history; /* Read-Planet-History file, or NULL: integrate the planets given */
+    uint32_t history_table_size; /* Planet-History-Max-Planet-Count (0 means 16) */
 } glx_exec_params;
 
-/* Create the loop: planets (n_bodies GM values, heliocentric r, v), n
- * particles (heliocentric pr, pv, ids).  Uploads, computes the initial
- * accelerations and flags, compacts, produces the first planet block. */
+/* Create the loop: planets (n_bodies GM values, heliocentric r, v; unused
+ * and may be NULL with a planet history), n particles (heliocentric pr,
+ * pv, ids).  Uploads, computes the initial accelerations and flags,
+ * compacts, produces the first planet block.  With a planet history the
+ * blocks follow its records (Executor::update_planets,
+ * src/executor.cu:226-299): each record interval of length T holds
+ * max(1, round(T / dt)) equal steps, at most tbsize per block. */
 int glx_exec_create(const glx_exec_params* p, uint32_t n_bodies, const double* m, const double* r, const double* v,
                     const uint32_t* planet_id, uint64_t n, const double* pr, const double* pv, const uint32_t* ids,
                     glx_exec** out);
@@ -345,8 +351,13 @@ int glx_exec_return(glx_exec* e, uint64_t n, const uint32_t* ids, const double* 
 /* Every particle slot (n_total): r, v (n x 3), flags, id, deathtime (time of
  * the flagging step, NaN if never flagged); any output may be NULL. */
 int glx_exec_download(glx_exec* e, double* r, double* v, uint16_t* flags, uint32_t* id, double* deathtime);
-/* Heliocentric planet positions ((n_bodies - 1) x 3) at the current time. */
+/* Heliocentric planet positions ((n_bodies - 1) x 3, n_bodies from
+ * glx_exec_bodies, at most GLX_MAX_BODIES) at the current time. */
 int glx_exec_planets(glx_exec* e, double* r);
+/* The bodies at the current time: count (Sun included), GM values and ids
+ * (any output may be NULL).  With a planet history the set can change
+ * between blocks. */
+int glx_exec_bodies(glx_exec* e, uint32_t* n_bodies, double* m, uint32_t* id);
 /* Device time of the last kernel and compaction, host time of the last planet block (ms). */
 int glx_exec_times(glx_exec* e, double* kernel_ms, double* partition_ms, double* planet_ms);
 

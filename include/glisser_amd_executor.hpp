@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "glisser_amd.h"
@@ -35,6 +36,12 @@ struct ExecutorConfig {
     bool resolve_encounters = false;  // Resolve-Encounters
     int form = GLX_FORM_FP64;         // kernel arithmetic
     int device = 0;
+    // Read-Planet-History: planets interpolated from this history file
+    // (blocks then follow its records: each interval of length T holds
+    // max(1, round(T / dt)) equal steps, at most tbsize per block) instead
+    // of being integrated; empty = integrate the planets given.
+    std::string planet_history;
+    uint32_t history_table_size = 16;  // Planet-History-Max-Planet-Count
 };
 
 // One particle handed to the host close-encounter queue: id, flags (0x01,
@@ -54,7 +61,8 @@ struct ParticleState {
 class Executor {
 public:
     // Planets: n_bodies GM values m (m[0] the Sun), heliocentric r, v
-    // (n_bodies x 3), ids (NULL: 0..n-1).  Particles: n heliocentric r, v
+    // (n_bodies x 3), ids (NULL: 0..n-1); not used (may be NULL) when
+    // cfg.planet_history names a history.  Particles: n heliocentric r, v
     // and ids.  Uploads everything, computes the initial accelerations and
     // flags, compacts, and produces the first planet block.
     Executor(const ExecutorConfig& cfg, uint32_t n_bodies, const double* m, const double* r, const double* v,
@@ -82,7 +90,12 @@ public:
     void return_particles(uint64_t n, const uint32_t* ids, const double* r, const double* v);
 
     ParticleState download() const;
-    // Heliocentric planet positions (n_bodies - 1 x 3) at time().
+    // The planets at time(): bodies (Sun first; the set can change between
+    // blocks of a planet history), their GM values and ids, and the
+    // heliocentric positions of bodies 1..n_bodies()-1 ((n_bodies() - 1) x 3).
+    uint32_t n_bodies() const;
+    std::vector<double> planet_masses() const;
+    std::vector<uint32_t> planet_ids() const;
     std::vector<double> planet_positions() const;
 
     // Device time of the last kernel and compaction, host time of the last
