@@ -58,6 +58,16 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
                     drift_fp64(p, A.dt, A.mu);
                     accel_fp64<NB>(p, rowp, A.pc, A.outer2, nb);
                     kick(p, A.half_dt);
+                } else if constexpr (NB > 0) {
+                    // the step's planet row is read from LDS before the drift,
+                    // so the read latency hides behind it
+                    double rowv[3 * NB];
+#pragma unroll
+                    for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
+                    kick_fma(p, A.half_dt);
+                    drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+                    accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowv, A.pc, A.outer2, nb);
+                    kick_fma(p, A.half_dt);
                 } else {
                     kick_fma(p, A.half_dt);
                     drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
