@@ -351,7 +351,7 @@ def main(argv=None):
     ap.add_argument("--waves-per-simd", type=int, default=3, help="resident-wave cap (0: none)")
     ap.add_argument("--launches-per-block", type=int, default=4,
                     help="launches each 2048-step block is split into (same results, shorter chain end)")
-    ap.add_argument("--cpu-sample", type=int, default=20000, help="particles in the CPU-baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=40000, help="particles in the CPU-baseline sample (~10 s)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--oracle-sample", type=int, default=256,
                     help="particles also run through the CPU oracle for the precision report (0: none)")

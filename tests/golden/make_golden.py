@@ -156,11 +156,45 @@ def plhist(ref):
     np.savez_compressed(d / "blocks.npz", **out)
 
 
+REFERENCE_INPUTS = {  # fixture name: file in the reference tree
+    "outer-planets.in": "data/outer-planets.in",
+    "three-body.in": "data/three-body.in",
+    "threebody-planets.state.in": "example/threebody-planets/state.in",
+    "threebody-planets.config.in": "example/threebody-planets/config.in",
+    "threebody-particles.state.in.full": "example/threebody-particles/state.in.full",
+    "threebody-particles.config.in": "example/threebody-particles/config.in",
+}
+
+
+def reference_inputs(ref):
+    """The reference's own input files (data files, copied unchanged) and what
+    the reference's readers make of them."""
+    import shutil
+    src = Path("/root/reference")
+    d = OUT / "reference_inputs"
+    d.mkdir(exist_ok=True)
+    for name, rel in REFERENCE_INPUTS.items():
+        shutil.copyfile(src / rel, d / name)
+    (d / "empty_particles.in").write_text("0\n")
+    out = {}
+    for name, fmt, ppath in (("threebody-planets.state.in", 0, None),
+                             ("threebody-particles.state.in.full", 0, None),
+                             ("outer-planets.in", 2, d / "empty_particles.in"),
+                             ("three-body.in", 2, d / "empty_particles.in")):
+        st = ref.state_load(d / name, fmt, particle_path=ppath)
+        for k, v in st.items():
+            out[f"{name}:{k}"] = v
+    np.savez_compressed(d / "loaded.npz", **out)
+    for name in ("threebody-planets.config.in", "threebody-particles.config.in"):
+        (d / (name + ".normalized")).write_text(ref.config_normalize((d / name).read_text()))
+
+
 if __name__ == "__main__":
     ref = Oracle(reference=True)
     parts = sys.argv[1:] or ["planets", "particle_block", "long_run", "encounters", "partition", "elements",
-                             "formats", "plhist"]
+                             "formats", "plhist", "reference_inputs"]
     for name in parts:
         globals()[name](ref)
-    for f in sorted(OUT.glob("*.npz")) + sorted((OUT / "formats").glob("*")) + sorted((OUT / "plhist").glob("*")):
+    for f in sorted(OUT.glob("*.npz")) + sorted((OUT / "formats").glob("*")) + sorted((OUT / "plhist").glob("*")) + \
+            sorted((OUT / "reference_inputs").glob("*")):
         print(f.name, f.stat().st_size)
