@@ -127,6 +127,8 @@ PlanetHistory::PlanetHistory(const std::string& path, double t0, double dt, uint
         oom_end_[k].x = read_le<float>(in_);
         oom_end_[k].y = read_le<float>(in_);
         oom_end_[k].z = read_le<float>(in_);
+        if (!in_) throw std::runtime_error("planet history: the record at t = " + std::to_string(t_end_) +
+                                           " is truncated");
         from_elements_M(m_[0] + gm_first_[k], aei_end_[k].x, aei_end_[k].y, aei_end_[k].z, oom_end_[k].x,
                         oom_end_[k].y, oom_end_[k].z, &r_[i], &v_[i]);
         id_[i] = pid;
@@ -171,6 +173,11 @@ void PlanetHistory::advance() {
         oom_end_[k].x = read_le<float>(in_);
         oom_end_[k].y = read_le<float>(in_);
         oom_end_[k].z = read_le<float>(in_);
+        if (!in_) {
+            ended_ = true;
+            throw std::runtime_error("planet history: the record at t = " + std::to_string(t_end_) +
+                                     " is truncated");
+        }
         if (std::isnan(aei_begin_[k].x)) continue;  // absent at the interval start
         Track tr;
         tr.id = pid;

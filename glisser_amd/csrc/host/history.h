@@ -61,7 +61,8 @@ public:
     // every planet id the file names).  Positions the reader on the interval
     // holding t0 and fills the planet table at t0.  Throws
     // std::runtime_error with the reference's messages: the file starts
-    // after t0, ends before it, or names more planets than the table holds.
+    // after t0, ends before it, or names more planets than the table holds;
+    // a record cut short is an error too (the reference would read past it).
     PlanetHistory(const std::string& path, double t0, double dt, uint32_t table_size);
 
     // The next block, at most tbsize steps; throws HistoryEnd at the end of

@@ -181,3 +181,21 @@ def test_errors(glx_lib, tmp_path):
         assert h.block(2048)["n_steps"] == 2
     with pytest.raises(native.GlxError, match="EOS reached"):
         h.block(2048)
+
+
+def test_truncated_record_is_an_error(glx_lib, tmp_path):
+    from glisser_amd import native
+    path = tmp_path / "h.bin"
+    write_plhist_bytes(path, GM_SUN_YR, sample_history(14, n_records=6))
+    data = path.read_bytes()
+    cut = tmp_path / "cut.bin"
+    cut.write_bytes(data[: 32 + 3 * (32 + 4 * 32) + 32 + 50])  # inside the fourth record's planets
+    h = native.PlanetHistory(cut, 0.0, 0.5)
+    h.block(2048)
+    h.block(2048)
+    with pytest.raises(native.GlxError, match="truncated"):
+        h.block(2048)
+    first = tmp_path / "cut0.bin"
+    first.write_bytes(data[: 32 + 32 + 40])  # inside the first record
+    with pytest.raises(native.GlxError, match="truncated"):
+        native.PlanetHistory(first, 0.0, 0.5)
