@@ -25,9 +25,9 @@
 //     reference-arithmetic drift, so results and flags stay defined for any
 //     input;
 //   * FMA contraction is allowed.
-// MIXED additionally starts each planet's 1/|dr| from the FP32 reciprocal
-// square root and refines it with one FP64 Newton step; everything else,
-// the accumulation included, stays FP64.
+// MIXED additionally starts each planet's r^-3 from the FP32 reciprocal
+// square root (corrected in FP64); everything else, the accumulation
+// included, stays FP64.
 #pragma once
 
 #include "wh_device.h"
@@ -195,12 +195,13 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
 
 // Planet sum and flag tests, FAST and MIXED forms.
 //
-// r^-3 of each planet: FAST takes 1/|dr| from v_rsq_f64 with one FP64
-// Newton step (error < 6e-15); MIXED starts from the FP32 reciprocal square
-// root (v_rsq_f32, ~1e-7) and takes the same FP64 Newton step (error
-// ~2e-14, still below the planet term's share of the step error), trading
-// the FP64 transcendental for two conversions.  The terms, their
-// accumulation, |dr|^2 and every test are FP64.
+// r^-3 of each planet: the hardware reciprocal square root y of |dr|^2
+// (FAST: v_rsq_f64; MIXED: v_rsq_f32 of the rounded |dr|^2, trading the
+// FP64 transcendental for two conversions) and the first-order correction
+// of its cube, y^3 (1 + 1.5 e) with e = 1 - |dr|^2 y^2, in FP64: relative
+// error 1.875 e^2, about 2e-14 (FAST) and 1e-13 (MIXED) at the estimates'
+// ~1e-7, far below the planet term's share of the step error.  The
+// terms, their accumulation, |dr|^2 and every test are FP64.
 //
 // The encounter, Sun and outer-boundary tests keep the reference's
 // precedence (src/wh.cu:151-170: the first planet inside its radius wins if
@@ -223,14 +224,18 @@ __device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict
         const double dz = p.rz - row[3 * j + 2];
         const double rad = fma(dz, dz, fma(dy, dy, dx * dx));
         close |= __ballot(rad < pc.rh2[j]);
+        // y ~ rad^-1/2 with relative error d; e = 1 - rad y^2 ~ -2d and
+        // rad^-3/2 = y^3 (1 - e)^-3/2 = y^3 (1 + 1.5 e + 1.875 e^2 + ...)
         double y;
         if constexpr (MIXED) {
             y = (double)__builtin_amdgcn_rsqf((float)rad);
-            y = fma(y, fma(-0.5 * rad, y * y, 0.5), y);
         } else {
-            y = rsqrt_nr1(rad);
+            y = __builtin_amdgcn_rsq(rad);
         }
-        const double fac = pc.m[j] * (y * y * y);
+        const double y2 = y * y;
+        const double e = fma(-rad, y2, 1.0);
+        const double my3 = pc.m[j] * (y2 * y);
+        const double fac = fma(my3, 1.5 * e, my3);
         ax = fma(-dx, fac, ax);
         ay = fma(-dy, fac, ay);
         az = fma(-dz, fac, az);

@@ -41,7 +41,7 @@ enum glx_status {
 enum glx_form {
     GLX_FORM_FP64 = 0,  /* reference arithmetic: fixed 5 Newton iterations, IEEE div/sqrt */
     GLX_FORM_FAST = 1,  /* FP64 throughout, fewer and cheaper operations (early-exit Newton) */
-    GLX_FORM_MIXED = 2  /* FAST drift; planet-kick 1/|dr| from FP32 rsqrt + one FP64 Newton step, FP64 sums */
+    GLX_FORM_MIXED = 2  /* FAST drift; planet r^-3 from the FP32 rsqrt, corrected in FP64; FP64 sums */
 };
 
 /* Partition modes of glx_partition(). */
