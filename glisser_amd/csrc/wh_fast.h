@@ -10,11 +10,12 @@
 //     the result feeds the state (relative error < 6e-15, a few ulp of the
 //     values it scales, far below the step's truncation error), none where
 //     it only scales a Newton correction;
-//   * sin/cos of the mean-anomaly step come from a five-term Taylor
-//     polynomial (|dM| <= 0.2); the first guess and each Newton correction
-//     rotate (sin E, cos E - 1) by the small angle with a short series
-//     instead of re-evaluating sin/cos; cos E - 1 and E - dM are carried
-//     explicitly, so f, g and the Kepler residual avoid cancellation;
+//   * sin/cos of the mean-anomaly step and of the start E come from a
+//     five-term Taylor polynomial (|dM|, |E| <= 0.2); each later Newton
+//     correction rotates (sin E, cos E - 1) by the small angle with a short
+//     series instead of re-evaluating sin/cos; cos E - 1 and E - dM are
+//     carried explicitly, so f, g and the Kepler residual avoid
+//     cancellation;
 //   * the Kepler solve starts from one Newton step off E = dM (instead of
 //     the reference's series guess) and takes one more correction (two in
 //     waves where some lane needs it), accepted when the error it can leave
@@ -56,8 +57,7 @@ __constant__ double kSinTaylor[5] = {-1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0
 __constant__ double kCosTaylor[5] = {-1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0, -1.0 / 3628800.0};
 
 // sin x and cos x - 1 from the first N Taylor terms (Horner in x^2).  With
-// N = 5 the truncation is below 9e-18 for |x| <= 0.2; with N = 4 below
-// 3e-20 for |x| <= 0.05.
+// N = 5 the truncation is below 9e-18 for |x| <= 0.2.
 template <int N>
 __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) {
     const double x2 = x * x;
@@ -72,23 +72,9 @@ __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) 
     cm1 = pc * x2;
 }
 
-// Advance (s, cm1) = (sin E, cos E - 1) to E + d (|d| <= 0.05; larger
-// corrections are routed to the fallback by the caller).
-__device__ __forceinline__ void rotate_taylor(double d, double& s, double& cm1) {
-    double sd, cd;
-    sincos_taylor<4>(d, sd, cd);
-    // sin(E+d) = s (1 + cd) + (1 + cm1) sd ; cos(E+d) - 1 = cm1 + cd + cm1 cd - s sd
-    const double s_new = s + sd + fma(s, cd, cm1 * sd);
-    const double c_new = cm1 + cd + fma(cm1, cd, -s * sd);
-    s = s_new;
-    cm1 = c_new;
-}
-
-// Validity bounds of the FAST drift's shortcuts: the five-term Taylor
-// sin/cos of dM (truncation below 9e-18 absolute at |dM| = 0.2) and the
-// four-term rotation by the start correction (below 3e-20 at 0.05).
+// Validity bound of the FAST drift's shortcut: the five-term Taylor sin/cos
+// of dM and of the start E (truncation below 9e-18 absolute at 0.2).
 constexpr double kMaxDM = 0.2;
-constexpr double kMaxStart = 0.05;
 
 // One Newton correction of the Kepler solve in the (dd, sin E, cos E - 1)
 // form; returns whether the error it leaves is provably below 1e-14
@@ -118,8 +104,8 @@ __device__ __forceinline__ bool newton_step(double esinEo, double ecosEo, double
 // p.r2 must hold |r|^2 (fma(z, z, fma(y, y, x * x))) of the current position.
 //
 // Straight-line path for the common case, then one wave vote: lanes for
-// which a shortcut does not hold (|dM| > 0.2, a start above 0.05, a solve
-// not provably within 1e-14 after two corrections, an unbound orbit) redo
+// which a shortcut does not hold (|dM| or the start |E| > 0.2, a solve not
+// provably within 1e-14 after two corrections, an unbound orbit) redo
 // the step with the reference-arithmetic drift from their saved state,
 // which also sets their 0x04 / 0x08 flags.
 __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, double inv_mu, double sqrt_mu,
@@ -154,8 +140,10 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     double f = fma(-esinEo, cm1, -ecosEo * s);
     double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     double dd = -f * __builtin_amdgcn_rcp(fp);
-    bool bad = !(ia > 0.0) || !(fabs(dM) <= kMaxDM) || !(fabs(dd) <= kMaxStart);
-    rotate_taylor(dd, s, cm1);
+    // sin / cos - 1 of the start E = dM + dd from the same polynomial
+    const double E1 = dM + dd;
+    bool bad = !(ia > 0.0) || !(fabs(dM) <= kMaxDM) || !(fabs(E1) <= kMaxDM);
+    sincos_taylor<5>(E1, s, cm1);
     // Newton corrections.  One leaves an error of at most |F''| d^2 / (2 F')
     // <= d^2 / (2 F') (|F''| <= e < 1), so the solve is accepted as soon as
     // that bound is below the reference's 1e-14 tolerance.  One correction
