@@ -10,17 +10,17 @@
 //     the result feeds the state (relative error < 6e-15, a few ulp of the
 //     values it scales, far below the step's truncation error), none where
 //     it only scales a Newton correction;
-//   * sin/cos of the mean-anomaly step and of the start E come from a
-//     five-term Taylor polynomial (|dM|, |E| <= 0.2); each later Newton
-//     correction rotates (sin E, cos E - 1) by the small angle with a short
-//     series instead of re-evaluating sin/cos; cos E - 1 and E - dM are
-//     carried explicitly, so f, g and the Kepler residual avoid
-//     cancellation;
-//   * the Kepler solve starts from one Newton step off E = dM (instead of
-//     the reference's series guess) and takes one more correction (two in
-//     waves where some lane needs it), accepted when the error it can leave
-//     is below the 1e-14 tolerance of the reference's convergence test
-//     (src/wh.cu:60-71);
+//   * the Kepler solve starts from the series reversion of the Kepler
+//     equation in E - E0 to third order (no sin/cos of the mean-anomaly
+//     step), evaluates sin/cos of that start with a five-term Taylor
+//     polynomial (|E| <= 0.2) and takes one short Newton correction
+//     (raw reciprocal, second-order rotation of (sin E, cos E - 1) by the
+//     correction), accepted only when the error it can leave is provably
+//     below one ulp; lanes outside that bound restart from sin/cos of the
+//     corrected E and take up to two full corrections, accepted when the
+//     error is below the 1e-14 tolerance of the reference's convergence
+//     test (src/wh.cu:60-71); cos E - 1 and E - dM are carried explicitly,
+//     so f, g and the Kepler residual avoid cancellation;
 //   * every shortcut is checked per lane; the few lanes outside them
 //     (large steps, slow convergence, unbound orbits) redo the step with the
 //     reference-arithmetic drift, so results and flags stay defined for any
@@ -72,20 +72,22 @@ __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) 
     cm1 = pc * x2;
 }
 
-// Validity bound of the FAST drift's shortcut: the five-term Taylor sin/cos
-// of dM and of the start E (truncation below 9e-18 absolute at 0.2).
+// Validity bound of the FAST drift's shortcut: |dM| (no full turns to
+// remove) and |E| wherever the five-term Taylor sin/cos is evaluated
+// (truncation below 9e-18 absolute at 0.2).
 constexpr double kMaxDM = 0.2;
 
 // One Newton correction of the Kepler solve in the (dd, sin E, cos E - 1)
 // form; returns whether the error it leaves is provably below 1e-14
 // (d^2 < 2e-14 F').  Corrections above 1e-3 are left unapplied and
 // reported as not converged (the two-term rotation is exact to rounding
-// only below that).
+// only below that).  r receives the reciprocal of F' it used.
 __device__ __forceinline__ bool newton_step(double esinEo, double ecosEo, double omec, double& dd, double& s,
-                                            double& cm1) {
+                                            double& cm1, double& r) {
     const double f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
     const double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
-    double d = -f * rcp_nr1(fp);
+    r = rcp_nr1(fp);
+    double d = -f * r;
     const bool small = fabs(d) <= 1e-3;
     d = small ? d : 0.0;
     dd += d;
@@ -100,14 +102,24 @@ __device__ __forceinline__ bool newton_step(double esinEo, double ecosEo, double
     return small && d2 < (2.0 * kTolKep) * fp;
 }
 
+// Acceptance bound of the first (short) Kepler correction: d^2 < 1e-20 F',
+// i.e. |d| < 1.5e-10 (F' < 2).  Within it the error the correction leaves
+// (d^2 / (2 F') < 5e-21), the raw reciprocal's share (|d| x 6e-8 < 1e-17)
+// and the terms the second-order rotation drops (s d^2 / 2, cm1 d^2 / 2,
+// d^3 / 6: below 3e-21 for |s| <= 0.2) are all under one ulp of the
+// quantities they touch.
+constexpr double kTolShort = 1e-20;
+
 // Kepler drift, FAST form.  inv_mu, sqrt_mu, inv_sqrt_mu: host constants;
 // p.r2 must hold |r|^2 (fma(z, z, fma(y, y, x * x))) of the current position.
 //
-// Straight-line path for the common case, then one wave vote: lanes for
-// which a shortcut does not hold (|dM| or the start |E| > 0.2, a solve not
-// provably within 1e-14 after two corrections, an unbound orbit) redo
-// the step with the reference-arithmetic drift from their saved state,
-// which also sets their 0x04 / 0x08 flags.
+// Straight-line path for the common case, then wave votes: lanes whose
+// first correction is not provably converged recompute sin / cos of E and
+// take up to two full corrections; lanes for which a shortcut does not
+// hold (|dM| or |E| > 0.2 where the polynomial is evaluated, a solve not
+// provably within 1e-14, an unbound orbit) redo the step with the
+// reference-arithmetic drift from their saved state, which also sets their
+// 0x04 / 0x08 flags.
 __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, double inv_mu, double sqrt_mu,
                                            double inv_sqrt_mu) {
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
@@ -116,50 +128,80 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     const double v2 = fma(vz0, vz0, fma(vy0, vy0, vx0 * vx0));
     const double vdotr = fma(vz0, rz0, fma(vy0, ry0, vx0 * rx0));
     const double idist = rsqrt_nr1(r2);
-    const double dist = r2 * idist;
     // 1/a = -2E/mu = 2/|r| - v^2/mu; bound orbits have 1/a > 0
     const double ia = fma(-v2, inv_mu, 2.0 * idist);
-    const double rs_ia = rsqrt_nr1(ia);              // sqrt(a)
+    const double rs_ia = rsqrt_nr1(ia);            // sqrt(a)
     const double a = rs_ia * rs_ia;
-    const double n_ = sqrt_mu * ia * (ia * rs_ia);  // sqrt(mu / a^3)
-    const double in_ = inv_sqrt_mu * a * rs_ia;     // 1/n
-    const double ecosEo = fma(-dist, ia, 1.0);
-    const double esinEo = vdotr * in_ * ia * ia;
+    const double sq_ia = ia * rs_ia;               // sqrt(1/a)
+    const double n_ = sqrt_mu * ia * sq_ia;        // sqrt(mu / a^3)
+    const double in_ = inv_sqrt_mu * a * rs_ia;    // 1/n
+    const double omec = r2 * idist * ia;           // 1 - e cos E0 = |r| / a
+    const double ecosEo = 1.0 - omec;
+    const double esinEo = vdotr * (inv_sqrt_mu * sq_ia);  // r.v / (n a^2)
+    const double q = a * idist;                    // a / |r| = 1 / (1 - e cos E0)
     // |dM| <= 0.2 on this path, so no full turns are removed and the time
     // left to advance, dM / n, is dt itself
     const double dM = dt * n_;
-    double s, cm1;
-    sincos_taylor<5>(dM, s, cm1);
-    const double omec = 1.0 - ecosEo;
     // Kepler residual F(E) = E - ecosEo sin E + esinEo (1 - cos E) - dM and
     // F'(E) = 1 - ecosEo cos E + esinEo sin E, written with dd = E - dM and
-    // (s, cm1) = (sin E, cos E - 1).  The start is one Newton step from
-    // E = dM (error ~ e^3 dM^2 / 2, against ~ e^2 dM for the reference's
-    // series guess); the raw reciprocal suffices there because the next
-    // Newton step corrects any error of the start quadratically.
-    double f = fma(-esinEo, cm1, -ecosEo * s);
-    double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
-    double dd = -f * __builtin_amdgcn_rcp(fp);
-    // sin / cos - 1 of the start E = dM + dd from the same polynomial
-    const double E1 = dM + dd;
+    // (s, cm1) = (sin E, cos E - 1).  Start: the series of F in E,
+    // (1 - c) E + (s0 / 2) E^2 + (c / 6) E^3 + O(E^4) = dM (c = ecosEo,
+    // s0 = esinEo), reverted to third order: with t = dM / (1 - c),
+    // A2 = s0 / (2 (1 - c)), A3 = c / (6 (1 - c)),
+    // E = t - A2 t^2 + (2 A2^2 - A3) t^3 + O(t^4).  Its error is ~ e t^4 / 100
+    // (below 3e-11 at t = 0.008, e = 0.1), so one correction usually ends
+    // the solve; any error of the start is corrected by the Newton steps and
+    // only decides how many run.
+    const double cq = ecosEo * q;
+    const double h = esinEo * q;
+    const double A2 = 0.5 * h;
+    const double b3 = fma(A2, h, cq * (-1.0 / 6.0));  // 2 A2^2 - A3
+    const double t = dM * q;
+    const double tu = t * fma(t, b3, -A2);
+    const double E1 = fma(tu, t, t);
+    // exact (Sterbenz) while E1 / dM lies in [1/2, 2], which holds for
+    // e < 1/3; otherwise its rounding is one more part of the start's error
+    double dd = E1 - dM;
     bool bad = !(ia > 0.0) || !(fabs(dM) <= kMaxDM) || !(fabs(E1) <= kMaxDM);
+    double s, cm1;
     sincos_taylor<5>(E1, s, cm1);
-    // Newton corrections.  One leaves an error of at most |F''| d^2 / (2 F')
-    // <= d^2 / (2 F') (|F''| <= e < 1), so the solve is accepted as soon as
-    // that bound is below the reference's 1e-14 tolerance.  One correction
-    // suffices when e dM is small (the config of most swarms); a second runs
-    // only in waves where some lane needs it.  The reciprocal with one
-    // Newton step (relative error ~2e-15) is exact enough for corrections
-    // that small.
-    bool conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1);
-    if (__any(!conv)) conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1);
+    // First correction, short form: raw reciprocal and the second-order
+    // rotation sin(E + d) = s + d (1 + cm1), cos(E + d) - 1 = cm1 - d (s + d/2),
+    // accepted only within kTolShort.  A Newton correction d leaves an error
+    // of at most |F''| d^2 / (2 F') <= d^2 / (2 F') (|F''| <= e < 1).
+    double f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
+    double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
+    double r = __builtin_amdgcn_rcp(fp);
+    const double d = -f * r;
+    dd += d;
+    const double s1 = fma(cm1, d, s + d);
+    cm1 = fma(fma(d, -0.5, -s), d, cm1);
+    s = s1;
+    bool conv = d * d < kTolShort * fp;
+    // Lanes outside that bound (large steps or eccentric orbits) restart from
+    // sin / cos of the corrected E and take full corrections (accurate
+    // reciprocal, third-order rotation, the reference's 1e-14 tolerance as a
+    // proven bound); these run only in waves where some lane needs them.
+    if (__any(!conv)) {
+        if (!conv) {
+            const double E2 = dM + dd;
+            bad |= !(fabs(E2) <= kMaxDM);
+            sincos_taylor<5>(E2, s, cm1);
+            conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1, r);
+        }
+        if (__any(!conv)) {
+            if (!conv) conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1, r);
+        }
+    }
     bad |= !conv;
     const double dE = dM + dd;
     fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
-    const double ifp = rcp_nr1(fp);
-    const double fg = fma(a * cm1, idist, 1.0);
+    // 1/F' at the solution: one Newton step from the reciprocal of the last
+    // correction's F', which differs from it by |d| e relative at most
+    const double ifp = fma(r, fma(-fp, r, 1.0), r);
+    const double fg = fma(cm1, q, 1.0);
     const double g = fma(s - dE, in_, dt);
-    const double fdot = -n_ * s * a * idist * ifp;
+    const double fdot = -(n_ * q) * s * ifp;
     const double gdot = fma(cm1, ifp, 1.0);
     p.rx = fma(rx0, fg, vx0 * g);
     p.ry = fma(ry0, fg, vy0 * g);

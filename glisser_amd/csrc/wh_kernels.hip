@@ -13,6 +13,36 @@
 
 namespace glx {
 
+// One step of one particle (skipped once it is flagged).  rowp: the step's
+// planet row in LDS; s: the step's index in the block.
+template <int NB, int FORM>
+__device__ __forceinline__ void wh_step(Particle& p, uint32_t& dti, const BlockArgs& A, const double* rowp,
+                                        uint32_t nb, uint32_t s) {
+    if (p.flags != 0) return;
+    if constexpr (FORM == GLX_FORM_FP64) {
+        kick(p, A.half_dt);
+        drift_fp64(p, A.dt, A.mu);
+        accel_fp64<NB>(p, rowp, A.pc, A.outer2, nb);
+        kick(p, A.half_dt);
+    } else if constexpr (NB > 0) {
+        // the step's planet row is read from LDS before the drift, so the
+        // read latency hides behind it
+        double rowv[3 * NB];
+#pragma unroll
+        for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
+        kick_fma(p, A.half_dt);
+        drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+        accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowv, A.pc, A.outer2, nb);
+        kick_fma(p, A.half_dt);
+    } else {
+        kick_fma(p, A.half_dt);
+        drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+        accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowp, A.pc, A.outer2, nb);
+        kick_fma(p, A.half_dt);
+    }
+    dti = A.step_offset + s + 1;
+}
+
 template <int NB, int FORM>
 __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
     extern __shared__ double lds[];
@@ -50,33 +80,7 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        for (uint32_t s = 0; s < cn; s++) {
-            if (p.flags == 0) {
-                const double* rowp = lds + s * row;
-                if constexpr (FORM == GLX_FORM_FP64) {
-                    kick(p, A.half_dt);
-                    drift_fp64(p, A.dt, A.mu);
-                    accel_fp64<NB>(p, rowp, A.pc, A.outer2, nb);
-                    kick(p, A.half_dt);
-                } else if constexpr (NB > 0) {
-                    // the step's planet row is read from LDS before the drift,
-                    // so the read latency hides behind it
-                    double rowv[3 * NB];
-#pragma unroll
-                    for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
-                    kick_fma(p, A.half_dt);
-                    drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-                    accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowv, A.pc, A.outer2, nb);
-                    kick_fma(p, A.half_dt);
-                } else {
-                    kick_fma(p, A.half_dt);
-                    drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-                    accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowp, A.pc, A.outer2, nb);
-                    kick_fma(p, A.half_dt);
-                }
-                dti = A.step_offset + cs + s + 1;
-            }
-        }
+        for (uint32_t s = 0; s < cn; s++) wh_step<NB, FORM>(p, dti, A, lds + s * row, nb, cs + s);
     }
 
     if (valid) {
