@@ -62,7 +62,9 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
         p.ay = B.d[7][i];
         p.az = B.d[8][i];
         p.flags = B.flags[i];
-        dti = p.flags == 0 ? base : B.dti[i];
+        // particles flagged before this launch keep their death step; the
+        // others end the launch at its last step unless flagged on the way
+        dti = p.flags == 0 ? base + B.steps : B.dti[i];
     } else {
         p.rx = p.ry = p.rz = p.vx = p.vy = p.vz = p.ax = p.ay = p.az = 0.0;
         p.flags = 0xFFFFu;  // padding lanes never step
@@ -76,23 +78,8 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        for (uint32_t s = 0; s < cn; s++) {
-            if (p.flags == 0) {
-                const double* rowp = lds + s * row;
-                if constexpr (FORM == GLX_FORM_FP64) {
-                    kick(p, S.half_dt);
-                    drift_fp64(p, S.dt, S.mu);
-                    accel_fp64<0, kBatchMaxBodies>(p, rowp, S.pc, S.outer2, nb);
-                    kick(p, S.half_dt);
-                } else {
-                    kick_fma(p, S.half_dt);
-                    drift_fast(p, S.dt, S.mu, S.inv_mu, S.sqrt_mu, S.inv_sqrt_mu);
-                    accel_vote<0, FORM == GLX_FORM_MIXED, kBatchMaxBodies>(p, rowp, S.pc, S.outer2, nb);
-                    kick_fma(p, S.half_dt);
-                }
-                dti = base + cs + s + 1;
-            }
-        }
+        for (uint32_t s = 0; s < cn; s++)
+            particle_step<0, FORM, kBatchMaxBodies>(p, dti, S, lds + s * row, nb, base + cs + s + 1);
     }
 
     if (valid) {

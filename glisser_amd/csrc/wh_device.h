@@ -74,22 +74,30 @@ struct Particle {
     uint32_t flags;
 };
 
-// Kepler drift in the reference's arithmetic (src/wh.cu:74-110 with kepeq at
-// src/wh.cu:42-72): fixed five Newton iterations, IEEE divides and square
-// roots, sin/cos from the device math library, and no FMA contraction, so
-// every basic operation rounds exactly as in the reference CPU build (the
-// particle orbits amplify a one-ulp difference to ~1e-10 over 1e5 steps).
-// An unbound orbit (energy >= 0) sets 0x08 and leaves r, v unchanged; the
+// f and g functions of the Kepler drift in the reference's arithmetic
+// (src/wh.cu:74-110 with kepeq at src/wh.cu:42-72): fixed five Newton
+// iterations, IEEE divides and square roots, sin/cos from the device math
+// library, and no FMA contraction, so every basic operation rounds exactly
+// as in the reference CPU build (the particle orbits amplify a one-ulp
+// difference to ~1e-10 over 1e5 steps).  The new state is
+// r = f r0 + g v0, v = fdot r0 + gdot v0.  Returns the flags of the drift:
+// 0x04 if the last Newton correction exceeds 1e-14; for an unbound orbit
+// (energy >= 0) 0x08 with the identity map, so r, v stay unchanged (the
 // reference GPU code computes a NaN orbit there instead, and its CPU twin
-// throws.
-__device__ __forceinline__ void drift_fp64(Particle& p, double dt, double mu) {
+// throws).
+__device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double rz, double vx, double vy,
+                                                   double vz, double dt, double mu, double& f, double& g,
+                                                   double& fdot, double& gdot) {
 #pragma clang fp contract(off)
-    const double dist = sqrt(p.rx * p.rx + p.ry * p.ry + p.rz * p.rz);
-    const double vdotr = p.vx * p.rx + p.vy * p.ry + p.vz * p.rz;
-    const double energy = (p.vx * p.vx + p.vy * p.vy + p.vz * p.vz) * 0.5 - mu / dist;
+    const double dist = sqrt(rx * rx + ry * ry + rz * rz);
+    const double vdotr = vx * rx + vy * ry + vz * rz;
+    const double energy = (vx * vx + vy * vy + vz * vz) * 0.5 - mu / dist;
     if (energy >= 0.0) {
-        p.flags = kFlagUnbound;
-        return;
+        f = 1.0;
+        g = 0.0;
+        fdot = 0.0;
+        gdot = 1.0;
+        return kFlagUnbound;
     }
     const double a = -0.5 * mu / energy;
     const double n_ = sqrt(mu / (a * a * a));
@@ -107,18 +115,27 @@ __device__ __forceinline__ void drift_fp64(Particle& p, double dt, double mu) {
     double delta = 0.0;
 #pragma unroll
     for (int k = 0; k < kKepIts; k++) {
-        const double f = dE - ecosEo * sindE + esinEo * (1. - cosdE) - dM;
-        const double fp = 1. - ecosEo * cosdE + esinEo * sindE;
-        delta = -f / fp;
+        const double fk = dE - ecosEo * sindE + esinEo * (1. - cosdE) - dM;
+        const double fpk = 1. - ecosEo * cosdE + esinEo * sindE;
+        delta = -fk / fpk;
         dE += delta;
         sincos(dE, &sindE, &cosdE);
     }
-    p.flags = fabs(delta) > kTolKep ? kFlagKepler : 0u;
     const double fp = 1.0 - ecosEo * cosdE + esinEo * sindE;
-    const double f = 1.0 + a * (cosdE - 1.0) / dist;
-    const double g = adv_dt + (sindE - dE) / n_;
-    const double fdot = -n_ * sindE * a / (dist * fp);
-    const double gdot = 1.0 + (cosdE - 1.0) / fp;
+    f = 1.0 + a * (cosdE - 1.0) / dist;
+    g = adv_dt + (sindE - dE) / n_;
+    fdot = -n_ * sindE * a / (dist * fp);
+    gdot = 1.0 + (cosdE - 1.0) / fp;
+    return fabs(delta) > kTolKep ? kFlagKepler : 0u;
+}
+
+// Kepler drift in the reference's arithmetic: kepler_fg_fp64 applied
+// without FMA contraction, as the reference does.
+__device__ __forceinline__ void drift_fp64(Particle& p, double dt, double mu) {
+#pragma clang fp contract(off)
+    double f, g, fdot, gdot;
+    p.flags = kepler_fg_fp64(p.rx, p.ry, p.rz, p.vx, p.vy, p.vz, dt, mu, f, g, fdot, gdot);
+    if (p.flags == kFlagUnbound) return;
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
     p.rx = rx0 * f + p.vx * g;
     p.ry = ry0 * f + p.vy * g;

@@ -35,6 +35,19 @@
 
 namespace glx {
 
+// Wave votes on a lane predicate, taken straight from its compare mask
+// (HIP's int-typed __any / __ballot move the predicate through a VGPR and
+// compare it back: two VALU instructions per vote).  Inactive lanes vote
+// false.
+__device__ __forceinline__ uint64_t wave_mask(bool pred) { return __builtin_amdgcn_ballot_w64(pred); }
+__device__ __forceinline__ bool wave_any(bool pred) { return __builtin_amdgcn_ballot_w64(pred) != 0; }
+// Empty register constraint: x is materialised here (an optimisation
+// barrier for its computation; no instruction is emitted).
+__device__ __forceinline__ void pin(double& x) { __asm__ volatile("" : "+v"(x)); }
+
+// This lane's bit of a wave mask.
+__device__ __forceinline__ bool lane_in(uint64_t mask) { return (mask >> __lane_id()) & 1; }
+
 // Reciprocal of x (x normal) with one Newton step from v_rcp_f64: the
 // estimate's relative error (< 6e-8) is squared, leaving < 3e-15 (measured
 // on gfx950: 2.2e-15).
@@ -117,11 +130,12 @@ constexpr double kTolShort = 1e-20;
 // first correction is not provably converged recompute sin / cos of E and
 // take up to two full corrections; lanes for which a shortcut does not
 // hold (|dM| or |E| > 0.2 where the polynomial is evaluated, a solve not
-// provably within 1e-14, an unbound orbit) redo the step with the
-// reference-arithmetic drift from their saved state, which also sets their
-// 0x04 / 0x08 flags.
-__device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, double inv_mu, double sqrt_mu,
-                                           double inv_sqrt_mu) {
+// provably within 1e-14, an unbound orbit) take the f and g functions of
+// the reference-arithmetic drift, which also sets their 0x04 / 0x08 flags.
+// Returns the wave mask of those lanes (the only ones whose flags the drift
+// can set).
+__device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu, double inv_mu,
+                                               double sqrt_mu, double inv_sqrt_mu) {
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
     const double vx0 = p.vx, vy0 = p.vy, vz0 = p.vz;
     const double r2 = p.r2;  // |r|^2, left by the previous step's planet sum
@@ -162,7 +176,9 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     // exact (Sterbenz) while E1 / dM lies in [1/2, 2], which holds for
     // e < 1/3; otherwise its rounding is one more part of the start's error
     double dd = E1 - dM;
-    bool bad = !(ia > 0.0) || !(fabs(dM) <= kMaxDM) || !(fabs(E1) <= kMaxDM);
+    // lanes for the reference-arithmetic drift, as a wave mask (one compare
+    // and a scalar OR per condition)
+    uint64_t bad = wave_mask(!(ia > 0.0)) | wave_mask(!(fabs(dM) <= kMaxDM)) | wave_mask(!(fabs(E1) <= kMaxDM));
     double s, cm1;
     sincos_taylor<5>(E1, s, cm1);
     // First correction, short form: raw reciprocal and the second-order
@@ -177,50 +193,52 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
     const double s1 = fma(cm1, d, s + d);
     cm1 = fma(fma(d, -0.5, -s), d, cm1);
     s = s1;
-    bool conv = d * d < kTolShort * fp;
+    const uint64_t retry = wave_mask(!(d * d < kTolShort * fp));
     // Lanes outside that bound (large steps or eccentric orbits) restart from
     // sin / cos of the corrected E and take full corrections (accurate
     // reciprocal, third-order rotation, the reference's 1e-14 tolerance as a
     // proven bound); these run only in waves where some lane needs them.
-    if (__any(!conv)) {
-        if (!conv) {
+    if (retry != 0) {
+        bool conv = true, far = false;
+        if (lane_in(retry)) {
             const double E2 = dM + dd;
-            bad |= !(fabs(E2) <= kMaxDM);
+            far = !(fabs(E2) <= kMaxDM);
             sincos_taylor<5>(E2, s, cm1);
             conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1, r);
         }
-        if (__any(!conv)) {
+        if (wave_any(!conv)) {
             if (!conv) conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1, r);
         }
+        bad |= wave_mask(far) | wave_mask(!conv);
     }
-    bad |= !conv;
     const double dE = dM + dd;
     fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     // 1/F' at the solution: one Newton step from the reciprocal of the last
     // correction's F', which differs from it by |d| e relative at most
     const double ifp = fma(r, fma(-fp, r, 1.0), r);
-    const double fg = fma(cm1, q, 1.0);
-    const double g = fma(s - dE, in_, dt);
-    const double fdot = -(n_ * q) * s * ifp;
-    const double gdot = fma(cm1, ifp, 1.0);
-    p.rx = fma(rx0, fg, vx0 * g);
-    p.ry = fma(ry0, fg, vy0 * g);
-    p.rz = fma(rz0, fg, vz0 * g);
+    double fg = fma(cm1, q, 1.0);
+    double g = fma(s - dE, in_, dt);
+    double fdot = -(n_ * q) * s * ifp;
+    double gdot = fma(cm1, ifp, 1.0);
+    // Lanes outside the shortcuts take the f and g functions of the
+    // reference-arithmetic drift (and its 0x04 / 0x08 flags) instead; the
+    // state itself is then updated once, in place, for every lane.
+    if (bad != 0) {
+        if (lane_in(bad)) p.flags = kepler_fg_fp64(rx0, ry0, rz0, vx0, vy0, vz0, dt, mu, fg, g, fdot, gdot);
+    }
     p.vx = fma(rx0, fdot, vx0 * gdot);
     p.vy = fma(ry0, fdot, vy0 * gdot);
     p.vz = fma(rz0, fdot, vz0 * gdot);
-    p.flags = 0;
-    if (__any(bad)) {
-        if (bad) {
-            p.rx = rx0;
-            p.ry = ry0;
-            p.rz = rz0;
-            p.vx = vx0;
-            p.vy = vy0;
-            p.vz = vz0;
-            drift_fp64(p, dt, mu);
-        }
-    }
+    p.rx = fma(rx0, fg, vx0 * g);
+    p.ry = fma(ry0, fg, vy0 * g);
+    p.rz = fma(rz0, fg, vz0 * g);
+    // Keep the velocity update here: left to the scheduler it sinks to the
+    // second kick, which keeps the old position live across the planet sum
+    // and costs a register copy per component every step.
+    pin(p.vx);
+    pin(p.vy);
+    pin(p.vz);
+    return bad;
 }
 
 // Planet sum and flag tests, FAST and MIXED forms.
@@ -239,10 +257,11 @@ __device__ __forceinline__ void drift_fast(Particle& p, double dt, double mu, do
 // overwrite), but the main path only ORs "some radius crossed" into one
 // wave vote; the exact assignment runs in the rare waves where a lane
 // crossed one.  |r|^2 of the new position is left in p.r2 for the next
-// drift (the kick in between does not move the particle).
+// drift (the kick in between does not move the particle).  Returns the
+// wave mask of lanes inside some radius (the only ones it can flag).
 template <int NB, bool MIXED, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES)>
-__device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict__ row, const PlanetConsts& pc,
-                                           double outer2, uint32_t nb) {
+__device__ __forceinline__ uint64_t accel_vote(Particle& p, const double* __restrict__ row, const PlanetConsts& pc,
+                                               double outer2, uint32_t nb) {
     double ax = row[0], ay = row[1], az = row[2];
     const uint32_t nbodies = NB > 0 ? (uint32_t)NB : nb;
     uint64_t close = 0;  // wave mask: lanes inside some planet's radius
@@ -253,7 +272,7 @@ __device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict
         const double dy = p.ry - row[3 * j + 1];
         const double dz = p.rz - row[3 * j + 2];
         const double rad = fma(dz, dz, fma(dy, dy, dx * dx));
-        close |= __ballot(rad < pc.rh2[j]);
+        close |= wave_mask(rad < pc.rh2[j]);
         // y ~ rad^-1/2 with relative error d; e = 1 - rad y^2 ~ -2d and
         // rad^-3/2 = y^3 (1 - e)^-3/2 = y^3 (1 + 1.5 e + 1.875 e^2 + ...)
         double y;
@@ -275,8 +294,9 @@ __device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict
     p.az = az;
     const double r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
     p.r2 = r2;
-    if ((close | __ballot(r2 < pc.rh2[0] || r2 > outer2)) != 0) {
-        if (((close >> __lane_id()) & 1) && p.flags == 0) {
+    const uint64_t event = close | wave_mask(r2 < pc.rh2[0]) | wave_mask(r2 > outer2);
+    if (event != 0) {
+        if (lane_in(close) && p.flags == 0) {
             for (uint32_t j = 1; j < nbodies; j++) {
                 const double dx = p.rx - row[3 * j];
                 const double dy = p.ry - row[3 * j + 1];
@@ -290,12 +310,52 @@ __device__ __forceinline__ void accel_vote(Particle& p, const double* __restrict
         if (r2 < pc.rh2[0]) p.flags = kFlagEncounter;
         if (r2 > outer2) p.flags = kFlagOut;
     }
+    return event;
 }
 
 __device__ __forceinline__ void kick_fma(Particle& p, double half_dt) {
     p.vx = fma(p.ax, half_dt, p.vx);
     p.vy = fma(p.ay, half_dt, p.vy);
     p.vz = fma(p.az, half_dt, p.vz);
+}
+
+// One step of one particle, any form (skipped once it is flagged).  C: the
+// launch's constants (dt, half_dt, mu, inv_mu, sqrt_mu, inv_sqrt_mu, pc,
+// outer2); rowp: the step's planet row in LDS; death: the death index a
+// particle flagged in this step records.  In the FAST / MIXED forms only
+// the rare branches that can set a flag (the reference-arithmetic drift, a
+// radius crossed) look at it.
+template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
+__device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const C& A, const double* rowp,
+                                              uint32_t nb, uint32_t death) {
+    if (p.flags != 0) return;
+    if constexpr (FORM == GLX_FORM_FP64) {
+        kick(p, A.half_dt);
+        drift_fp64(p, A.dt, A.mu);
+        accel_fp64<NB, NMAX>(p, rowp, A.pc, A.outer2, nb);
+        kick(p, A.half_dt);
+        if (p.flags != 0) dti = death;
+    } else {
+        uint64_t event;
+        if constexpr (NB > 0) {
+            // the step's planet row is read from LDS before the drift, so
+            // the read latency hides behind it
+            double rowv[3 * NB];
+#pragma unroll
+            for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
+            kick_fma(p, A.half_dt);
+            event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+            event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, A.outer2, nb);
+        } else {
+            kick_fma(p, A.half_dt);
+            event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+            event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, A.outer2, nb);
+        }
+        kick_fma(p, A.half_dt);
+        if (__builtin_expect(event != 0, 0)) {
+            if (p.flags != 0) dti = death;
+        }
+    }
 }
 
 }  // namespace glx

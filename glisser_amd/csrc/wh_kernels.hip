@@ -13,36 +13,6 @@
 
 namespace glx {
 
-// One step of one particle (skipped once it is flagged).  rowp: the step's
-// planet row in LDS; s: the step's index in the block.
-template <int NB, int FORM>
-__device__ __forceinline__ void wh_step(Particle& p, uint32_t& dti, const BlockArgs& A, const double* rowp,
-                                        uint32_t nb, uint32_t s) {
-    if (p.flags != 0) return;
-    if constexpr (FORM == GLX_FORM_FP64) {
-        kick(p, A.half_dt);
-        drift_fp64(p, A.dt, A.mu);
-        accel_fp64<NB>(p, rowp, A.pc, A.outer2, nb);
-        kick(p, A.half_dt);
-    } else if constexpr (NB > 0) {
-        // the step's planet row is read from LDS before the drift, so the
-        // read latency hides behind it
-        double rowv[3 * NB];
-#pragma unroll
-        for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
-        kick_fma(p, A.half_dt);
-        drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-        accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowv, A.pc, A.outer2, nb);
-        kick_fma(p, A.half_dt);
-    } else {
-        kick_fma(p, A.half_dt);
-        drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-        accel_vote<NB, FORM == GLX_FORM_MIXED>(p, rowp, A.pc, A.outer2, nb);
-        kick_fma(p, A.half_dt);
-    }
-    dti = A.step_offset + s + 1;
-}
-
 template <int NB, int FORM>
 __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
     extern __shared__ double lds[];
@@ -64,9 +34,10 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         p.ay = A.ay[i];
         p.az = A.az[i];
         p.flags = A.flags[i];
-        // particles flagged before this block keep their death step;
-        // the others count steps from the block's offset in the launch
-        dti = p.flags == 0 ? A.step_offset : A.dti[i];
+        // particles flagged before this block keep their death step; the
+        // others end the block at its last step unless they are flagged
+        // on the way
+        dti = p.flags == 0 ? A.step_offset + A.n_steps : A.dti[i];
     } else {
         p.rx = p.ry = p.rz = p.vx = p.vy = p.vz = p.ax = p.ay = p.az = 0.0;
         p.flags = 0xFFFFu;  // padding lanes never step
@@ -80,7 +51,8 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        for (uint32_t s = 0; s < cn; s++) wh_step<NB, FORM>(p, dti, A, lds + s * row, nb, cs + s);
+        for (uint32_t s = 0; s < cn; s++)
+            particle_step<NB, FORM>(p, dti, A, lds + s * row, nb, A.step_offset + cs + s + 1);
     }
 
     if (valid) {
