@@ -319,12 +319,40 @@ __device__ __forceinline__ void kick_fma(Particle& p, double half_dt) {
     p.vz = fma(p.az, half_dt, p.vz);
 }
 
-// One step of one particle, any form (skipped once it is flagged).  C: the
-// launch's constants (dt, half_dt, mu, inv_mu, sqrt_mu, inv_sqrt_mu, pc,
-// outer2); rowp: the step's planet row in LDS; death: the death index a
-// particle flagged in this step records.  In the FAST / MIXED forms only
-// the rare branches that can set a flag (the reference-arithmetic drift, a
-// radius crossed) look at it.
+// One FAST / MIXED step of every active lane (none of them flagged).  C:
+// the launch's constants (dt, half_dt, mu, inv_mu, sqrt_mu, inv_sqrt_mu,
+// pc, outer2); rowp: the step's planet row in LDS; death: the death index a
+// particle flagged in this step records.  Only the rare branches that can
+// set a flag (the reference-arithmetic drift, a radius crossed) look at
+// the flags.  Returns whether some lane of the wave was flagged.
+template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
+__device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A, const double* rowp, uint32_t nb,
+                                          uint32_t death) {
+    uint64_t event;
+    if constexpr (NB > 0) {
+        // the step's planet row is read from LDS before the drift, so the
+        // read latency hides behind it
+        double rowv[3 * NB];
+#pragma unroll
+        for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
+        kick_fma(p, A.half_dt);
+        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, A.outer2, nb);
+    } else {
+        kick_fma(p, A.half_dt);
+        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, A.outer2, nb);
+    }
+    kick_fma(p, A.half_dt);
+    if (__builtin_expect(event != 0, 0)) {
+        if (p.flags != 0) dti = death;
+        return wave_mask(p.flags != 0) != 0;
+    }
+    return false;
+}
+
+// One step of one particle, any form (skipped once it is flagged); the
+// arguments are those of fast_step.
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
 __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const C& A, const double* rowp,
                                               uint32_t nb, uint32_t death) {
@@ -336,26 +364,34 @@ __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const 
         kick(p, A.half_dt);
         if (p.flags != 0) dti = death;
     } else {
-        uint64_t event;
-        if constexpr (NB > 0) {
-            // the step's planet row is read from LDS before the drift, so
-            // the read latency hides behind it
-            double rowv[3 * NB];
-#pragma unroll
-            for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
-            kick_fma(p, A.half_dt);
-            event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-            event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, A.outer2, nb);
-        } else {
-            kick_fma(p, A.half_dt);
-            event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-            event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, A.outer2, nb);
-        }
-        kick_fma(p, A.half_dt);
-        if (__builtin_expect(event != 0, 0)) {
-            if (p.flags != 0) dti = death;
+        fast_step<NB, FORM, NMAX>(p, dti, A, rowp, nb, death);
+    }
+}
+
+// Steps [0, cn) of a staged chunk whose rows start at `rows` (row: doubles
+// per step); death0 + s + 1 is step s's death index.  In the FAST / MIXED
+// forms a wave with no flagged lane steps without the per-lane skip, two
+// steps per iteration: with neither the skip's merge nor a loop-carried
+// copy, the state registers of consecutive steps alternate.  Once one of
+// its lanes is flagged (and for an odd last step) the wave finishes the
+// chunk step by step.
+template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
+__device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C& A, const double* rows,
+                                            uint32_t row, uint32_t cn, uint32_t nb, uint32_t death0) {
+    uint32_t s = 0;
+    if constexpr (FORM != GLX_FORM_FP64) {
+        if (wave_mask(p.flags != 0) == 0) {
+            while (s + 1 < cn) {
+                bool flagged = fast_step<NB, FORM, NMAX>(p, dti, A, rows + s * row, nb, death0 + s + 1);
+                s++;
+                if (flagged) break;
+                flagged = fast_step<NB, FORM, NMAX>(p, dti, A, rows + s * row, nb, death0 + s + 1);
+                s++;
+                if (flagged) break;
+            }
         }
     }
+    for (; s < cn; s++) particle_step<NB, FORM, NMAX>(p, dti, A, rows + s * row, nb, death0 + s + 1);
 }
 
 }  // namespace glx

@@ -51,8 +51,7 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        for (uint32_t s = 0; s < cn; s++)
-            particle_step<NB, FORM>(p, dti, A, lds + s * row, nb, A.step_offset + cs + s + 1);
+        chunk_steps<NB, FORM>(p, dti, A, lds, row, cn, nb, A.step_offset + cs);
     }
 
     if (valid) {
