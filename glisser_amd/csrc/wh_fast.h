@@ -89,6 +89,9 @@ __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) 
 // remove) and |E| wherever the five-term Taylor sin/cos is evaluated
 // (truncation below 9e-18 absolute at 0.2).
 constexpr double kMaxDM = 0.2;
+// Bound of the start |E| for the three-term Taylor sin/cos (truncation below
+// 1e-18 absolute at 0.02: x^9 / 9! for sin, x^8 / 8! for cos - 1).
+constexpr double kMaxStart = 0.02;
 
 // One Newton correction of the Kepler solve in the (dd, sin E, cos E - 1)
 // form; returns whether the error it leaves is provably below 1e-14
@@ -178,9 +181,12 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     double dd = E1 - dM;
     // lanes for the reference-arithmetic drift, as a wave mask (one compare
     // and a scalar OR per condition)
-    uint64_t bad = wave_mask(!(ia > 0.0)) | wave_mask(!(fabs(dM) <= kMaxDM)) | wave_mask(!(fabs(E1) <= kMaxDM));
+    uint64_t bad = wave_mask(!(ia > 0.0)) | wave_mask(!(fabs(dM) <= kMaxDM));
+    // sin / cos - 1 of the start: three Taylor terms suffice for |E1| <= 0.02
+    // (the regime of the long-period swarms this path is built for); a lane
+    // with a larger start takes the retry below, which evaluates five terms
     double s, cm1;
-    sincos_taylor<5>(E1, s, cm1);
+    sincos_taylor<3>(E1, s, cm1);
     // First correction, short form: raw reciprocal and the second-order
     // rotation sin(E + d) = s + d (1 + cm1), cos(E + d) - 1 = cm1 - d (s + d/2),
     // accepted only within kTolShort.  A Newton correction d leaves an error
@@ -193,7 +199,7 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     const double s1 = fma(cm1, d, s + d);
     cm1 = fma(fma(d, -0.5, -s), d, cm1);
     s = s1;
-    const uint64_t retry = wave_mask(!(d * d < kTolShort * fp));
+    const uint64_t retry = wave_mask(!(d * d < kTolShort * fp)) | wave_mask(!(fabs(E1) <= kMaxStart));
     // Lanes outside that bound (large steps or eccentric orbits) restart from
     // sin / cos of the corrected E and take full corrections (accurate
     // reciprocal, third-order rotation, the reference's 1e-14 tolerance as a
@@ -283,8 +289,8 @@ __device__ __forceinline__ uint64_t accel_vote(Particle& p, const double* __rest
         }
         const double y2 = y * y;
         const double e = fma(-rad, y2, 1.0);
-        const double my3 = pc.m[j] * (y2 * y);
-        const double fac = fma(my3, 1.5 * e, my3);
+        // m (1 + 1.5 e): 1.5 m is loop-invariant and stays in a register
+        const double fac = (y2 * y) * fma(e, 1.5 * pc.m[j], pc.m[j]);
         ax = fma(-dx, fac, ax);
         ay = fma(-dy, fac, ay);
         az = fma(-dz, fac, az);
