@@ -48,14 +48,14 @@ HEADLINE_FORM = "mixed"
 # summed over the block kernel and divided by its particle-steps); see
 # profiles/wh_block_200k_counters.md.  Used with the live rate to report
 # the counter-based share of FP64 vector peak.
-COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 283.0, "mixed": 279.0}
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 232.0, "mixed": 228.0}
 
 # HBM bytes per time-block of the mixed form at this workload and launch
 # shape (4 slices x 4 launches per block: each launch reads and writes its
 # slice's state once), from rocprofv3 FETCH_SIZE (x 2.0, the factor a copy
 # kernel of known size needs) and WRITE_SIZE (x 1.0); see
 # profiles/wh_block_200k_counters.md.
-TRAFFIC_BYTES_PER_BLOCK = {"mixed": 131.1e6}
+TRAFFIC_BYTES_PER_BLOCK = {"mixed": 132.4e6}
 
 
 def copy_probe(n: int = 1_000_000, repeats: int = 3) -> None:
