@@ -78,7 +78,7 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        chunk_steps<0, FORM, kBatchMaxBodies>(p, dti, S, lds, row, cn, nb, base + cs);
+        chunk_steps<0, FORM, kBatchMaxBodies>(p, dti, S, S.pc.m, S.pc.rh2, lds, row, cn, nb, base + cs);
     }
 
     if (valid) {

@@ -63,23 +63,26 @@ __device__ __forceinline__ double rsqrt_nr1(double x) {
     return fma(y, fma(-0.5 * x, y * y, 0.5), y);
 }
 
-// Taylor coefficients of sin (x^3..x^11) and of cos - 1 (x^2..x^10):
-// (-1)^k / n!.  Kept in constant memory so they reach the FMAs as scalar
-// operands (literal FP64 constants would be copied into VGPRs first).
-__constant__ double kSinTaylor[5] = {-1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0 / 362880.0, -1.0 / 39916800.0};
-__constant__ double kCosTaylor[5] = {-1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0, -1.0 / 3628800.0};
+// Taylor coefficients of sin (x^3..x^11, row 0) and of cos - 1 (x^2..x^10,
+// row 1): (-1)^k / n!.  Kept in constant memory so they reach the FMAs as
+// scalar operands (literal FP64 constants would be copied into VGPRs first).
+__constant__ double kTaylor[2][5] = {
+    {-1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0 / 362880.0, -1.0 / 39916800.0},
+    {-1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0, -1.0 / 3628800.0}};
 
-// sin x and cos x - 1 from the first N Taylor terms (Horner in x^2).  With
-// N = 5 the truncation is below 9e-18 for |x| <= 0.2.
+// sin x and cos x - 1 from the first N Taylor terms (Horner in x^2), with
+// the coefficient table c (kTaylor).  With N = 5 the truncation is below
+// 9e-18 for |x| <= 0.2.
 template <int N>
-__device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1) {
+__device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1,
+                                              const double (*__restrict__ c)[5] = kTaylor) {
     const double x2 = x * x;
-    double ps = kSinTaylor[N - 1];
-    double pc = kCosTaylor[N - 1];
+    double ps = c[0][N - 1];
+    double pc = c[1][N - 1];
 #pragma unroll
     for (int k = N - 2; k >= 0; k--) {
-        ps = fma(ps, x2, kSinTaylor[k]);
-        pc = fma(pc, x2, kCosTaylor[k]);
+        ps = fma(ps, x2, c[0][k]);
+        pc = fma(pc, x2, c[1][k]);
     }
     s = fma(ps * x2, x, x);
     cm1 = pc * x2;
@@ -263,10 +266,13 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
 // overwrite), but the main path only ORs "some radius crossed" into one
 // wave vote; the exact assignment runs in the rare waves where a lane
 // crossed one.  |r|^2 of the new position is left in p.r2 for the next
-// drift (the kick in between does not move the particle).  Returns the
-// wave mask of lanes inside some radius (the only ones it can flag).
+// drift (the kick in between does not move the particle).  hm, hrh2: the
+// GM and squared radii read on the main path (the values of pc.m, pc.rh2).
+// Returns the wave mask of lanes inside some radius (the only ones it can
+// flag).
 template <int NB, bool MIXED, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES)>
 __device__ __forceinline__ uint64_t accel_vote(Particle& p, const double* __restrict__ row, const PlanetConsts& pc,
+                                               const double* __restrict__ hm, const double* __restrict__ hrh2,
                                                double outer2, uint32_t nb) {
     double ax = row[0], ay = row[1], az = row[2];
     const uint32_t nbodies = NB > 0 ? (uint32_t)NB : nb;
@@ -278,7 +284,7 @@ __device__ __forceinline__ uint64_t accel_vote(Particle& p, const double* __rest
         const double dy = p.ry - row[3 * j + 1];
         const double dz = p.rz - row[3 * j + 2];
         const double rad = fma(dz, dz, fma(dy, dy, dx * dx));
-        close |= wave_mask(rad < pc.rh2[j]);
+        close |= wave_mask(rad < hrh2[j]);
         // y ~ rad^-1/2 with relative error d; e = 1 - rad y^2 ~ -2d and
         // rad^-3/2 = y^3 (1 - e)^-3/2 = y^3 (1 + 1.5 e + 1.875 e^2 + ...)
         double y;
@@ -290,7 +296,7 @@ __device__ __forceinline__ uint64_t accel_vote(Particle& p, const double* __rest
         const double y2 = y * y;
         const double e = fma(-rad, y2, 1.0);
         // m (1 + 1.5 e): 1.5 m is loop-invariant and stays in a register
-        const double fac = (y2 * y) * fma(e, 1.5 * pc.m[j], pc.m[j]);
+        const double fac = (y2 * y) * fma(e, 1.5 * hm[j], hm[j]);
         ax = fma(-dx, fac, ax);
         ay = fma(-dy, fac, ay);
         az = fma(-dz, fac, az);
@@ -300,7 +306,7 @@ __device__ __forceinline__ uint64_t accel_vote(Particle& p, const double* __rest
     p.az = az;
     const double r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
     p.r2 = r2;
-    const uint64_t event = close | wave_mask(r2 < pc.rh2[0]) | wave_mask(r2 > outer2);
+    const uint64_t event = close | wave_mask(r2 < hrh2[0]) | wave_mask(r2 > outer2);
     if (event != 0) {
         if (lane_in(close) && p.flags == 0) {
             for (uint32_t j = 1; j < nbodies; j++) {
@@ -327,13 +333,15 @@ __device__ __forceinline__ void kick_fma(Particle& p, double half_dt) {
 
 // One FAST / MIXED step of every active lane (none of them flagged).  C:
 // the launch's constants (dt, half_dt, mu, inv_mu, sqrt_mu, inv_sqrt_mu,
-// pc, outer2); rowp: the step's planet row in LDS; death: the death index a
-// particle flagged in this step records.  Only the rare branches that can
+// pc, outer2); hm, hrh2: the bodies' GM and squared radii as the planet sum
+// reads them every step (pc's own arrays, or copies held in registers);
+// rowp: the step's planet row in LDS; death: the death index a particle
+// flagged in this step records.  Only the rare branches that can
 // set a flag (the reference-arithmetic drift, a radius crossed) look at
 // the flags.  Returns whether some lane of the wave was flagged.
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
-__device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A, const double* rowp, uint32_t nb,
-                                          uint32_t death) {
+__device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A, const double* hm,
+                                          const double* hrh2, const double* rowp, uint32_t nb, uint32_t death) {
     uint64_t event;
     if constexpr (NB > 0) {
         // the step's planet row is read from LDS before the drift, so the
@@ -343,11 +351,11 @@ __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A
         for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
         kick_fma(p, A.half_dt);
         event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, A.outer2, nb);
+        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, hm, hrh2, A.outer2, nb);
     } else {
         kick_fma(p, A.half_dt);
         event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, A.outer2, nb);
+        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, hm, hrh2, A.outer2, nb);
     }
     kick_fma(p, A.half_dt);
     if (__builtin_expect(event != 0, 0)) {
@@ -360,8 +368,9 @@ __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A
 // One step of one particle, any form (skipped once it is flagged); the
 // arguments are those of fast_step.
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
-__device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const C& A, const double* rowp,
-                                              uint32_t nb, uint32_t death) {
+__device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const C& A, const double* hm,
+                                              const double* hrh2, const double* rowp, uint32_t nb,
+                                              uint32_t death) {
     if (p.flags != 0) return;
     if constexpr (FORM == GLX_FORM_FP64) {
         kick(p, A.half_dt);
@@ -370,7 +379,7 @@ __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const 
         kick(p, A.half_dt);
         if (p.flags != 0) dti = death;
     } else {
-        fast_step<NB, FORM, NMAX>(p, dti, A, rowp, nb, death);
+        fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rowp, nb, death);
     }
 }
 
@@ -382,22 +391,23 @@ __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const 
 // its lanes is flagged (and for an odd last step) the wave finishes the
 // chunk step by step.
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
-__device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C& A, const double* rows,
-                                            uint32_t row, uint32_t cn, uint32_t nb, uint32_t death0) {
+__device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C& A, const double* hm,
+                                            const double* hrh2, const double* rows, uint32_t row, uint32_t cn,
+                                            uint32_t nb, uint32_t death0) {
     uint32_t s = 0;
     if constexpr (FORM != GLX_FORM_FP64) {
         if (wave_mask(p.flags != 0) == 0) {
             while (s + 1 < cn) {
-                bool flagged = fast_step<NB, FORM, NMAX>(p, dti, A, rows + s * row, nb, death0 + s + 1);
+                bool flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
                 s++;
                 if (flagged) break;
-                flagged = fast_step<NB, FORM, NMAX>(p, dti, A, rows + s * row, nb, death0 + s + 1);
+                flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
                 s++;
                 if (flagged) break;
             }
         }
     }
-    for (; s < cn; s++) particle_step<NB, FORM, NMAX>(p, dti, A, rows + s * row, nb, death0 + s + 1);
+    for (; s < cn; s++) particle_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
 }
 
 }  // namespace glx

@@ -13,6 +13,10 @@
 
 namespace glx {
 
+// Bodies whose per-step constants the specialised FAST / MIXED kernels keep
+// in vector registers.
+constexpr int kRegBodies = 5;
+
 template <int NB, int FORM>
 __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
     extern __shared__ double lds[];
@@ -44,6 +48,28 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
     }
     p.r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
 
+    // With a compile-time body count of at most kRegBodies, the FAST / MIXED
+    // forms hold the bodies' GM and squared radii in vector registers: the
+    // scalar file is full with the launch's other constants, and a value
+    // that overflows it is read back every step.  (More bodies would push
+    // the kernel past the registers of three waves per SIMD.)
+    constexpr bool kInRegs = NB > 0 && NB <= kRegBodies && FORM != GLX_FORM_FP64;
+    constexpr int kH = kInRegs ? NB : 1;
+    double hm[kH], hrh2[kH];
+    const double* pm = A.pc.m;
+    const double* prh2 = A.pc.rh2;
+    if constexpr (kInRegs) {
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            hm[j] = A.pc.m[j];
+            hrh2[j] = A.pc.rh2[j];
+            pin(hm[j]);
+            pin(hrh2[j]);
+        }
+        pm = hm;
+        prh2 = hrh2;
+    }
+
     for (uint32_t cs = 0; cs < A.n_steps; cs += A.chunk) {
         const uint32_t cn = min(A.chunk, A.n_steps - cs);
         __syncthreads();  // every wave is done with the previous chunk
@@ -51,7 +77,7 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        chunk_steps<NB, FORM>(p, dti, A, lds, row, cn, nb, A.step_offset + cs);
+        chunk_steps<NB, FORM>(p, dti, A, pm, prh2, lds, row, cn, nb, A.step_offset + cs);
     }
 
     if (valid) {
