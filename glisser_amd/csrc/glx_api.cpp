@@ -20,6 +20,7 @@
 
 #include "errors.h"
 #include "glisser_amd.h"
+#include "staging.h"
 #include "launch.h"
 
 using glx::BlockArgs;
@@ -736,28 +737,10 @@ int glx_accel_range(glx_ctx* c, uint64_t begin, uint64_t n, uint32_t n_bodies, c
 
 int glx_pack_encounters(glx_ctx* c, uint64_t begin, uint64_t n, double t_block, glx_encounter* out) {
     if (!c) return fail(GLX_EINVAL, "null context");
-    if (int rc = check_range(c, begin, n)) return rc;
-    if (n == 0) return GLX_OK;
-    if (!out) return fail(GLX_EINVAL, "null output");
-    GLX_HIP(hipSetDevice(c->device));
-    if (n > c->enc_cap) {
-        GLX_HIP(hipStreamSynchronize(c->main_s));
-        if (c->enc_dev) (void)hipFree(c->enc_dev);
-        if (c->enc_pin) (void)hipHostFree(c->enc_pin);
-        c->enc_dev = nullptr;
-        c->enc_pin = nullptr;
-        c->enc_cap = 0;
-        const uint64_t cap = std::max<uint64_t>(n, 4096);
-        if (hipMalloc((void**)&c->enc_dev, cap * sizeof(glx_encounter)) != hipSuccess ||
-            hipHostMalloc((void**)&c->enc_pin, cap * sizeof(glx_encounter), hipHostMallocDefault) != hipSuccess)
-            return fail(GLX_ENOMEM, "encounter staging of %llu records failed", (unsigned long long)cap);
-        c->enc_cap = cap;
-    }
-    GLX_HIP(glx::launch_pack_encounters(c->cur, c->keep_rb ? &c->rb : nullptr, (uint32_t)begin, (uint32_t)n, t_block,
-                                        c->enc_dev, c->main_s));
-    GLX_HIP(hipMemcpyAsync(c->enc_pin, c->enc_dev, n * sizeof(glx_encounter), hipMemcpyDeviceToHost, c->main_s));
-    GLX_HIP(hipStreamSynchronize(c->main_s));
-    std::memcpy(out, c->enc_pin, n * sizeof(glx_encounter));
+    if (n > 0 && !out) return fail(GLX_EINVAL, "null output");
+    const glx_encounter* staged = nullptr;
+    if (int rc = glx::pack_encounters_staged(c, begin, n, t_block, &staged)) return rc;
+    if (n > 0) std::memcpy(out, staged, n * sizeof(glx_encounter));
     return GLX_OK;
 }
 
@@ -926,3 +909,35 @@ int glx_track_pack(glx_ctx* c, uint32_t n_planets, const double* pm, const doubl
 }
 
 }  // extern "C"
+
+namespace glx {
+
+int pack_encounters_staged(glx_ctx* c, uint64_t begin, uint64_t n, double t_block, const glx_encounter** out) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (!out) return fail(GLX_EINVAL, "null output");
+    *out = nullptr;
+    if (int rc = check_range(c, begin, n)) return rc;
+    if (n == 0) return GLX_OK;
+    GLX_HIP(hipSetDevice(c->device));
+    if (n > c->enc_cap) {
+        GLX_HIP(hipStreamSynchronize(c->main_s));
+        if (c->enc_dev) (void)hipFree(c->enc_dev);
+        if (c->enc_pin) (void)hipHostFree(c->enc_pin);
+        c->enc_dev = nullptr;
+        c->enc_pin = nullptr;
+        c->enc_cap = 0;
+        const uint64_t cap = std::max<uint64_t>(n, 4096);
+        if (hipMalloc((void**)&c->enc_dev, cap * sizeof(glx_encounter)) != hipSuccess ||
+            hipHostMalloc((void**)&c->enc_pin, cap * sizeof(glx_encounter), hipHostMallocDefault) != hipSuccess)
+            return fail(GLX_ENOMEM, "encounter staging of %llu records failed", (unsigned long long)cap);
+        c->enc_cap = cap;
+    }
+    GLX_HIP(glx::launch_pack_encounters(c->cur, c->keep_rb ? &c->rb : nullptr, (uint32_t)begin, (uint32_t)n, t_block,
+                                        c->enc_dev, c->main_s));
+    GLX_HIP(hipMemcpyAsync(c->enc_pin, c->enc_dev, n * sizeof(glx_encounter), hipMemcpyDeviceToHost, c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->main_s));
+    *out = c->enc_pin;
+    return GLX_OK;
+}
+
+}  // namespace glx

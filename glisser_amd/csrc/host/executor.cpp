@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../errors.h"
+#include "../staging.h"
 #include "glisser_amd.h"
 #include "glisser_amd_executor.hpp"
 #include "history.h"
@@ -165,10 +166,12 @@ struct Executor::Impl {
     void record_flagged(uint64_t begin, uint64_t end, uint64_t n_enc, double t_block, double dt_block) {
         const uint64_t n = end - begin;
         if (n == 0) return;
-        std::vector<glx_encounter> rec(n);
-        check(glx_pack_encounters(ctx, begin, n, t_block, rec.data()), "glx_pack_encounters");
+        // the records are read where the device copy put them (pinned
+        // staging), not copied through a temporary
+        const glx_encounter* rec = nullptr;
+        check(pack_encounters_staged(ctx, begin, n, t_block, &rec), "glx_pack_encounters");
         for (uint64_t k = 0; k < n; k++) set_deathtime(rec[k].id, t_block + (double)rec[k].step * dt_block);
-        queue.insert(queue.end(), rec.begin(), rec.begin() + n_enc);
+        queue.insert(queue.end(), rec, rec + n_enc);
     }
 
     // Stable compaction of [0, n) (kill first unless resolving); returns

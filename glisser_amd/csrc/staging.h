@@ -1,0 +1,16 @@
+// Library-internal access to a context's pinned staging buffers, for the
+// C++ host loop in the same library (not part of the C ABI).
+#pragma once
+
+#include <cstdint>
+
+#include "glisser_amd.h"
+
+namespace glx {
+
+// glx_pack_encounters without the final copy: packs particles
+// [begin, begin + n) of the context into its pinned staging buffer and sets
+// *out to it.  The records stay valid until the next pack on this context.
+int pack_encounters_staged(glx_ctx* c, uint64_t begin, uint64_t n, double t_block, const glx_encounter** out);
+
+}  // namespace glx
