@@ -241,12 +241,6 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     p.rx = fma(rx0, fg, vx0 * g);
     p.ry = fma(ry0, fg, vy0 * g);
     p.rz = fma(rz0, fg, vz0 * g);
-    // Keep the velocity update here: left to the scheduler it sinks to the
-    // second kick, which keeps the old position live across the planet sum
-    // and costs a register copy per component every step.
-    pin(p.vx);
-    pin(p.vy);
-    pin(p.vz);
     return bad;
 }
 
