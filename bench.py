@@ -57,7 +57,7 @@ COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 232.0, "mixed": 228.0}
 # slice's state once), from rocprofv3 FETCH_SIZE (x 2.0, the factor a copy
 # kernel of known size needs) and WRITE_SIZE (x 1.0); see
 # profiles/wh_block_200k_counters.md.
-TRAFFIC_BYTES_PER_BLOCK = {"mixed": 132.4e6}
+TRAFFIC_BYTES_PER_BLOCK = {"mixed": 133.1e6}
 
 
 def copy_probe(n: int = 1_000_000, repeats: int = 3) -> None:
@@ -74,6 +74,14 @@ def copy_probe(n: int = 1_000_000, repeats: int = 3) -> None:
             ctx.snapshot_rollback()
         ctx.sync()
     print(json.dumps({"copy_probe_bytes_read": 58 * n, "copy_probe_bytes_written": 58 * n, "repeats": repeats}))
+
+
+def waves_per_simd(args, form: str) -> int:
+    """Resident-wave cap of a form's launch: --waves-per-simd if given, else
+    3 for fp64 and none for the fast forms (see the option's note)."""
+    if args.waves_per_simd is not None:
+        return args.waves_per_simd
+    return 3 if form == "fp64" else 0
 
 
 def flops_per_particle_step(n_planets: int) -> int:
@@ -353,7 +361,12 @@ def main(argv=None):
     ap.add_argument("--particles", type=int, default=N_PARTICLES,
                     help="swarm size (the benchmark workload is 200,000)")
     ap.add_argument("--shards", type=int, default=4, help="particle slices, one stream each")
-    ap.add_argument("--waves-per-simd", type=int, default=3, help="resident-wave cap (0: none)")
+    # Resident-wave cap through the LDS request.  The fast forms use >= 147
+    # VGPRs, which already limits them to 3 waves per SIMD; without the cap
+    # they stage 273-step chunks in 32 KB (3 % faster than the capped 455-step
+    # chunks on the same box).  The fp64 form (89 VGPRs) is 4.5 % faster capped.
+    ap.add_argument("--waves-per-simd", type=int, default=None,
+                    help="resident-wave cap (0: none; default: 3 for fp64, none for fast and mixed)")
     ap.add_argument("--launches-per-block", type=int, default=4,
                     help="launches each 2048-step block is split into (same results, shorter chain end)")
     ap.add_argument("--cpu-sample", type=int, default=40000, help="particles in the CPU-baseline sample (~10 s)")
@@ -387,7 +400,8 @@ def main(argv=None):
     workgroups = (args.particles + 255) // 256
     per_form = {}
     for form in forms:
-        r = run_gpu(inputs, form, args.steps, args.warmup, args.shards, args.waves_per_simd, args.launches_per_block)
+        r = run_gpu(inputs, form, args.steps, args.warmup, args.shards, waves_per_simd(args, form),
+                    args.launches_per_block)
         finals[form] = r.pop("final")
         block_s = r["kernel_ms"] / 1e3 / args.steps
         per_block = r["particle_steps"] / args.steps
@@ -427,7 +441,7 @@ def main(argv=None):
             "form": head,
             "blocks_per_launch": args.steps,
             "shards": args.shards,
-            "waves_per_simd": args.waves_per_simd,
+            "waves_per_simd": waves_per_simd(args, head),
             "launches_per_block": args.launches_per_block,
             "planet_log": f"device cubic-Hermite interpolation of a {HIST_SPACING:g}-d history",
             "seed": args.seed,
