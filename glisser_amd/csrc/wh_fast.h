@@ -12,19 +12,22 @@
 //     it only scales a Newton correction;
 //   * the Kepler solve starts from the series reversion of the Kepler
 //     equation in E - E0 to third order (no sin/cos of the mean-anomaly
-//     step), evaluates sin/cos of that start with a five-term Taylor
-//     polynomial (|E| <= 0.2) and takes one short Newton correction
-//     (raw reciprocal, second-order rotation of (sin E, cos E - 1) by the
-//     correction), accepted only when the error it can leave is provably
-//     below one ulp; lanes outside that bound restart from sin/cos of the
-//     corrected E and take up to two full corrections, accepted when the
+//     step), evaluates sin/cos of that start with a three-term Taylor
+//     polynomial (|E| <= 0.02: long-period orbits) and takes one short
+//     Newton correction (raw reciprocal, second-order rotation of
+//     (sin E, cos E - 1) by the correction), accepted only when the error it
+//     can leave is provably below one ulp; lanes outside that bound, or with
+//     a larger start, restart from a five-term polynomial at the corrected E
+//     (|E| <= 0.2) and take up to two full corrections, accepted when the
 //     error is below the 1e-14 tolerance of the reference's convergence
 //     test (src/wh.cu:60-71); cos E - 1 and E - dM are carried explicitly,
 //     so f, g and the Kepler residual avoid cancellation;
-//   * every shortcut is checked per lane; the few lanes outside them
-//     (large steps, slow convergence, unbound orbits) redo the step with the
-//     reference-arithmetic drift, so results and flags stay defined for any
-//     input;
+//   * every shortcut is checked per lane, and every choice that changes a
+//     lane's arithmetic is made per lane (a particle's result does not
+//     depend on its wave-mates); the few lanes outside the shortcuts
+//     (large steps, slow convergence, unbound orbits) take the f and g
+//     functions of the reference-arithmetic drift, so results and flags
+//     stay defined for any input;
 //   * FMA contraction is allowed.
 // MIXED additionally starts each planet's r^-3 from the FP32 reciprocal
 // square root (corrected in FP64); everything else, the accumulation
