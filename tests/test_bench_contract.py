@@ -1,0 +1,36 @@
+"""bench.py's contract pieces that run without a GPU: the algorithmic FLOP
+count it prices the roofline with, the refusal of more than one GPU, and the
+per-form launch defaults."""
+import argparse
+import subprocess
+import sys
+
+import pytest
+
+from tests.conftest import ROOT
+
+
+@pytest.fixture(scope="module")
+def bench():
+    sys.path.insert(0, str(ROOT))
+    import bench as b
+    return b
+
+
+def test_flops_per_particle_step(bench):
+    # (135 + 16 n_p) basic + (16 + n_p) div + (2 + n_p) sqrt + 14 sin/cos
+    assert bench.flops_per_particle_step(4) == 199 + 20 + 6 + 14 == 239
+    assert bench.flops_per_particle_step(1) == 151 + 17 + 3 + 14
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr + r.stdout and not r.stdout.startswith("{")
+
+
+def test_wave_cap_defaults(bench):
+    auto = argparse.Namespace(waves_per_simd=None)
+    assert bench.waves_per_simd(auto, "fp64") == 3
+    assert bench.waves_per_simd(auto, "fast") == 0 and bench.waves_per_simd(auto, "mixed") == 0
+    assert bench.waves_per_simd(argparse.Namespace(waves_per_simd=4), "mixed") == 4
