@@ -14,7 +14,7 @@
 //     equation in E - E0 to third order (no sin/cos of the mean-anomaly
 //     step), evaluates sin/cos of that start with a three-term Taylor
 //     polynomial (|E| <= 0.02: long-period orbits) and takes one short
-//     Newton correction (raw reciprocal, second-order rotation of
+//     Newton correction (raw reciprocal, first-order rotation of
 //     (sin E, cos E - 1) by the correction), accepted only when the error it
 //     can leave is provably below one ulp; lanes outside that bound, or with
 //     a larger start, restart from a five-term polynomial at the corrected E
@@ -127,9 +127,9 @@ __device__ __forceinline__ bool newton_step(double esinEo, double ecosEo, double
 // Acceptance bound of the first (short) Kepler correction: d^2 < 1e-20 F',
 // i.e. |d| < 1.5e-10 (F' < 2).  Within it the error the correction leaves
 // (d^2 / (2 F') < 5e-21), the raw reciprocal's share (|d| x 6e-8 < 1e-17)
-// and the terms the second-order rotation drops (s d^2 / 2, cm1 d^2 / 2,
-// d^3 / 6: below 3e-21 for |s| <= 0.2) are all under one ulp of the
-// quantities they touch.
+// and the terms its rotation drops (s d^2 / 2 and d^3 / 6 from sin, d^2 / 2
+// from cos - 1: below 1.2e-20) are all under one ulp of the quantities they
+// touch (cos E - 1 only enters f, g-dot and F', all of order one).
 constexpr double kTolShort = 1e-20;
 
 // Kepler drift, FAST form.  inv_mu, sqrt_mu, inv_sqrt_mu: host constants;
@@ -193,17 +193,17 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     // with a larger start takes the retry below, which evaluates five terms
     double s, cm1;
     sincos_taylor<3>(E1, s, cm1);
-    // First correction, short form: raw reciprocal and the second-order
-    // rotation sin(E + d) = s + d (1 + cm1), cos(E + d) - 1 = cm1 - d (s + d/2),
-    // accepted only within kTolShort.  A Newton correction d leaves an error
-    // of at most |F''| d^2 / (2 F') <= d^2 / (2 F') (|F''| <= e < 1).
+    // First correction, short form: raw reciprocal and the rotation
+    // sin(E + d) = s + d (1 + cm1), cos(E + d) - 1 = cm1 - s d, accepted only
+    // within kTolShort.  A Newton correction d leaves an error of at most
+    // |F''| d^2 / (2 F') <= d^2 / (2 F') (|F''| <= e < 1).
     double f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
     double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     double r = __builtin_amdgcn_rcp(fp);
     const double d = -f * r;
     dd += d;
     const double s1 = fma(cm1, d, s + d);
-    cm1 = fma(fma(d, -0.5, -s), d, cm1);
+    cm1 = fma(-s, d, cm1);
     s = s1;
     const uint64_t retry = wave_mask(!(d * d < kTolShort * fp)) | wave_mask(!(fabs(E1) <= kMaxStart));
     // Lanes outside that bound (large steps or eccentric orbits) restart from
