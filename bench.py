@@ -264,8 +264,10 @@ def run_encounter(forms, n_particles: int, n_blocks: int, seed: int) -> dict:
     i < 10 deg; encounter radius 3.5 Hill radii; Resolve-Encounters on, so
     every block is rollback snapshot -> kernel -> stable three-way compaction
     -> encounter records (block-start and flagging states) to the host queue,
-    with the next planet block produced on the host meanwhile (glx::Executor).
-    Timed: n_blocks blocks after one warm-up block.  Also reports the
+    with the next planet block produced and the previous block's records
+    queued on the host meanwhile (glx::Executor).  Timed: one call running
+    n_blocks blocks after one warm-up block; particle-steps counted from each
+    particle's death time.  Also reports the
     Jacobi-constant drift of the particles never flagged."""
     from glisser_amd import native, systems
 
@@ -289,26 +291,26 @@ def run_encounter(forms, n_particles: int, n_blocks: int, seed: int) -> dict:
                              resolve_encounters=True, form=form) as ex:
             ex.step(1)
             queued0 = len(ex.take_encounters()["id"])
-            n0 = ex.status()["n_alive"]
-            el = 0.0
-            handed = 0
-            psteps = 0
-            other_flagged = 0
-            for _ in range(n_blocks):
-                alive = ex.status()["n_alive"]
-                t0 = time.perf_counter()
-                ex.step(1)  # the records are in the host queue when this returns
-                el += time.perf_counter() - t0
-                rec = ex.take_encounters()
-                handed += len(rec["id"])
-                alive_now = ex.status()["n_alive"]
-                # particle-steps: full blocks of the particles still unflagged plus the partial
-                # blocks of those handed to the queue; particles flagged otherwise (outer limit,
-                # none expected in this swarm) are counted as zero steps
-                psteps += alive_now * TBSIZE + int(rec["step"].astype(np.int64).sum())
-                other_flagged += int(alive - alive_now - len(rec["id"]))
+            st = ex.status()
+            n0, t_s = st["n_alive"], st["t"]
+            # one call for all timed blocks: each block's records are taken
+            # on the host while the next block's kernel runs
+            t0 = time.perf_counter()
+            ex.step(n_blocks)  # the records are in the host queue when this returns
+            el = time.perf_counter() - t0
+            rec = ex.take_encounters()
+            handed = len(rec["id"])
+            st = ex.status()
+            t_e = st["t"]
+            # particle-steps, exactly: every particle alive at t_s ran until its
+            # death time (the end of its flagging step) or to t_e
+            dd = ex.download()
+            death = np.where(np.isnan(dd["deathtime"]), t_e, np.minimum(dd["deathtime"], t_e))
+            live = np.isnan(dd["deathtime"]) | (dd["deathtime"] > t_s)
+            psteps = int(np.rint((death[live] - t_s) / DT).sum())
+            other_flagged = int(n0 - st["n_alive"] - handed)
             kernel_ms, part_ms, planet_ms = ex.times()
-            d = ex.download()
+            d = dd
             rp = ex.planet_positions()[0]
         vp = np.cross([0.0, 0.0, 1.0], rp) * np.sqrt((s.m[0] + s.m[1]) / a_p**3)
         order = np.argsort(d["id"])

@@ -160,18 +160,64 @@ struct Executor::Impl {
         row_h0.assign(b.log_h0.begin() + (size_t)(b.n_steps - 1) * 3, b.log_h0.begin() + (size_t)b.n_steps * 3);
     }
 
-    // Records particles [begin, end) flagged during the block that started at
-    // t_block; in resolve mode the encounter class [begin, begin + n_enc)
-    // also goes to the queue with its rollback state.
-    void record_flagged(uint64_t begin, uint64_t end, uint64_t n_enc, double t_block, double dt_block) {
+    // Records of the particles a block flagged, packed on the device into
+    // the context's pinned staging buffer and not yet moved to the queue and
+    // the death-time table (the next block's kernel runs meanwhile).
+    const glx_encounter* staged = nullptr;
+    uint64_t staged_n = 0, staged_enc = 0;
+    double staged_t = 0.0, staged_dt = 0.0;
+
+    // Stages the records of particles [begin, end) flagged during the block
+    // that started at t_block; in resolve mode the first n_enc of them (the
+    // encounter class) go to the queue with their rollback state.
+    void stage_flagged(uint64_t begin, uint64_t end, uint64_t n_enc, double t_block, double dt_block) {
         const uint64_t n = end - begin;
         if (n == 0) return;
-        // the records are read where the device copy put them (pinned
-        // staging), not copied through a temporary
-        const glx_encounter* rec = nullptr;
-        check(pack_encounters_staged(ctx, begin, n, t_block, &rec), "glx_pack_encounters");
-        for (uint64_t k = 0; k < n; k++) set_deathtime(rec[k].id, t_block + (double)rec[k].step * dt_block);
-        queue.insert(queue.end(), rec, rec + n_enc);
+        check(pack_encounters_staged(ctx, begin, n, t_block, &staged), "glx_pack_encounters");
+        staged_n = n;
+        staged_enc = n_enc;
+        staged_t = t_block;
+        staged_dt = dt_block;
+    }
+
+    // Moves staged records to the death-time table and the queue, read where
+    // the device copy put them.
+    void take_staged() {
+        for (uint64_t k = 0; k < staged_n; k++)
+            set_deathtime(staged[k].id, staged_t + (double)staged[k].step * staged_dt);
+        queue.insert(queue.end(), staged, staged + staged_enc);
+        staged = nullptr;
+        staged_n = staged_enc = 0;
+    }
+
+    // One time-block: its kernel; then, on the host while the kernel runs,
+    // the records the previous block staged and the next planet block; then
+    // the compaction, this block's records staged, and the next planet block
+    // uploaded.
+    void block() {
+        if (next.n_steps == 0) {
+            take_staged();
+            throw std::runtime_error("planet history ended at t = " + std::to_string(t) + ": " + history_end);
+        }
+        const uint64_t prev = n_alive;
+        const double t_block = next.t0, dt_block = next.dt;
+        const uint32_t n_steps = next.n_steps;
+        check(glx_set_counts(ctx, prev, n_total), "glx_set_counts");
+        if (prev) {
+            if (cfg.resolve_encounters) check(glx_snapshot_rollback(ctx), "glx_snapshot_rollback");
+            check(glx_run_block(ctx, buf, dt_block, cfg.outer_radius, cfg.form), "glx_run_block");
+        }
+        take_staged();
+        // the planets at the end of this block, then the next block
+        take_row(next);
+        produce_block(t_block + (double)n_steps * dt_block);
+        if (prev) {
+            const uint64_t ne = compact(prev);
+            stage_flagged(n_alive, prev, cfg.resolve_encounters ? ne : 0, t_block, dt_block);
+        }
+        buf ^= 1;
+        upload_block();
+        t = next.n_steps ? next.t0 : t_block + (double)n_steps * dt_block;
     }
 
     // Stable compaction of [0, n) (kill first unless resolving); returns
@@ -275,7 +321,8 @@ Executor::Executor(const ExecutorConfig& cfg, uint32_t n_bodies, const double* m
               "glx_init_accel");
         if (cfg.resolve_encounters) check(glx_snapshot_rollback(s.ctx), "glx_snapshot_rollback");
         const uint64_t ne = s.compact(n);
-        s.record_flagged(s.n_alive, n, cfg.resolve_encounters ? ne : 0, cfg.t0, cfg.dt);
+        s.stage_flagged(s.n_alive, n, cfg.resolve_encounters ? ne : 0, cfg.t0, cfg.dt);
+        s.take_staged();
     }
     s.produce_block(cfg.t0);
     s.upload_block();
@@ -284,33 +331,19 @@ Executor::Executor(const ExecutorConfig& cfg, uint32_t n_bodies, const double* m
 
 Executor::~Executor() = default;
 
-void Executor::step() {
-    Impl& s = *p_;
-    if (s.next.n_steps == 0) throw std::runtime_error("planet history ended at t = " + std::to_string(s.t) + ": " +
-                                                      s.history_end);
-    const uint64_t prev = s.n_alive;
-    const double t_block = s.next.t0, dt_block = s.next.dt;
-    const uint32_t n_steps = s.next.n_steps;
-    check(glx_set_counts(s.ctx, prev, s.n_total), "glx_set_counts");
-    if (prev) {
-        if (s.cfg.resolve_encounters) check(glx_snapshot_rollback(s.ctx), "glx_snapshot_rollback");
-        check(glx_run_block(s.ctx, s.buf, dt_block, s.cfg.outer_radius, s.cfg.form), "glx_run_block");
-    }
-    // the planets at the end of this block, then the next block (the host
-    // work overlaps the kernel)
-    s.take_row(s.next);
-    s.produce_block(t_block + (double)n_steps * dt_block);
-    if (prev) {
-        const uint64_t ne = s.compact(prev);
-        s.record_flagged(s.n_alive, prev, s.cfg.resolve_encounters ? ne : 0, t_block, dt_block);
-    }
-    s.buf ^= 1;
-    s.upload_block();
-    s.t = s.next.n_steps ? s.next.t0 : t_block + (double)n_steps * dt_block;
-}
+void Executor::step() { run(1); }
 
+// Blocks in a row: each block's records are taken while the next block's
+// kernel runs; all of them are in the queue when this returns.
 void Executor::run(uint32_t n_blocks) {
-    for (uint32_t k = 0; k < n_blocks; k++) step();
+    Impl& s = *p_;
+    try {
+        for (uint32_t k = 0; k < n_blocks; k++) s.block();
+    } catch (...) {
+        s.take_staged();
+        throw;
+    }
+    s.take_staged();
 }
 
 double Executor::time() const { return p_->t; }
