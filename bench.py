@@ -50,7 +50,7 @@ HEADLINE_FORM = "mixed"
 # summed over the block kernel and divided by its particle-steps); see
 # profiles/wh_block_200k_counters.md.  Used with the live rate to report
 # the counter-based share of FP64 vector peak.
-COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 232.0, "mixed": 228.0}
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 230.0, "mixed": 226.0}
 
 # HBM bytes per time-block of the mixed form at this workload and launch
 # shape (4 slices x 4 launches per block: each launch reads and writes its
