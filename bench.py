@@ -452,6 +452,8 @@ def main(argv=None):
         "forms": per_form,
         "roofline": {
             "bound": "fp64-valu",
+            "bound_note": ("FP64 vector (VALU) issue: no matrix-core work on this path; MI355X's FP64 vector "
+                           "peak equals its dense FP64 MFMA peak, 78.6 TFLOP/s"),
             "achieved": h["alg_tflops"],
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
