@@ -347,7 +347,30 @@ def cpu_baseline(inputs, sample_particles: int):
     steps = int(out["dti"].astype(np.int64).sum())
     return {"value": steps / el, "unit": "particle-steps/s", "cores": 1, "kind": kind,
             "sample": f"{n} particles x 1 block of {TBSIZE} steps of the same swarm and planet log "
-                      f"({steps} particle-steps, {el:.1f} s, single thread)"}
+                      f"({steps} particle-steps, {el:.1f} s, single thread)",
+            "host": host_description()}
+
+
+def host_description() -> dict:
+    """The CPU the baseline ran on: model, logical CPUs, C library, and the
+    compiler and flags of the CPU build (oracle/Makefile)."""
+    import platform
+    import subprocess
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cxx = subprocess.run(["g++", "--version"], capture_output=True, text=True, timeout=10).stdout.splitlines()[0]
+    except (OSError, IndexError, subprocess.SubprocessError):
+        cxx = "unknown"
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "libc": " ".join(platform.libc_ver()),
+            "compiler": f"{cxx}; -O3 -ffp-contract=off"}
 
 
 def main(argv=None):
