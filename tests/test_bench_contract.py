@@ -1,6 +1,7 @@
 """bench.py's contract pieces that run without a GPU: the algorithmic FLOP
-count it prices the roofline with, the refusal of more than one GPU, and the
-per-form launch defaults."""
+count it prices the roofline with, the refusal of more than one GPU, the
+per-form launch defaults and the CPU baseline's single-thread and all-cores
+legs."""
 import argparse
 import subprocess
 import sys
@@ -34,3 +35,16 @@ def test_wave_cap_defaults(bench):
     assert bench.waves_per_simd(auto, "fp64") == 3
     assert bench.waves_per_simd(auto, "fast") == 0 and bench.waves_per_simd(auto, "mixed") == 0
     assert bench.waves_per_simd(argparse.Namespace(waves_per_simd=4), "mixed") == 4
+
+
+def test_cpu_baseline_slices_cover_the_swarm(bench):
+    # the all-cores leg splits the swarm into disjoint slices: every particle
+    # of this all-alive swarm is counted once, TBSIZE steps each
+    inputs = bench.make_inputs(96, 1, 3)
+    r = bench.cpu_baseline(inputs, 40, runs=2, threads=3)
+    assert r["cores"] == 1 and r["kind"] in ("reference", "port") and r["value"] > 0
+    assert f"({40 * bench.TBSIZE} particle-steps, median of 2 runs" in r["sample"]
+    ac = r["all_cores"]
+    assert ac["cores"] == 3 and ac["value"] > 0
+    assert f"({96 * bench.TBSIZE} particle-steps" in ac["sample"]
+    assert set(r["host"]) == {"cpu_model", "logical_cpus", "libc", "compiler"}
