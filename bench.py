@@ -21,6 +21,13 @@ metric names; fast is within 1-2 % of it); the others are reported beside
 it, with their orbital-element differences from the fp64 form at the end
 of the run.
 
+Beside the headline: a batched launch of 64 systems (9.45M particles), a
+1M-particle three-body run that hands encounter records to the host, and
+the track output of the 200k swarm (orbital elements formed on the device
+as the reference's 28-byte records, then written to a track file).  The
+CPU leg (skipped by --no-cpu) times the reference's CPU path on the host,
+and runs the oracle over a particle sample as the checker.
+
 Prints one JSON line.  --gpus N with N > 1 exits: this build uses one GPU.
 """
 from __future__ import annotations
@@ -58,6 +65,8 @@ COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 230.0, "mixed": 226.0}
 # kernel of known size needs) and WRITE_SIZE (x 1.0); see
 # profiles/wh_block_200k_counters.md.
 TRAFFIC_BYTES_PER_BLOCK = {"mixed": 133.1e6}
+
+TRACK_RECORD_BYTES = 28  # u32 id + 6 float32 per particle in a track snapshot
 
 
 def copy_probe(n: int = 1_000_000, repeats: int = 3) -> None:
@@ -326,6 +335,79 @@ def run_encounter(forms, n_particles: int, n_blocks: int, seed: int) -> dict:
     return out
 
 
+def run_track(inputs, repeats: int = 5, seed: int = 5) -> dict:
+    """Track output of the whole swarm, the reference's particle-history pass
+    (targets/main.cpp:148-156 sort_by_id, src/data.cpp:770-812
+    save_binary_track): each particle's orbital elements about the planets'
+    barycentre, formed on the device as 28-byte records in id order
+    (glx_track_pack: elements, id ranking, record pack, copy to host memory),
+    then one snapshot appended to a track file (glx_track_write_records).
+    The particles sit in the context in a shuffled slot order, as after
+    compactions.  Both calls are synchronous: host clock, median of `repeats`
+    after one untimed call.  Returns the records under "records"."""
+    import tempfile
+    from glisser_amd import native
+
+    s = inputs[0]
+    n = s.n
+    order = np.random.default_rng(seed).permutation(n)
+    planets = {"pm": s.m, "pr": s.r, "pv": s.v, "pid": np.arange(s.n_bodies, dtype=np.uint32)}
+    with native.Context(n, s.n_bodies, 16) as ctx, tempfile.TemporaryDirectory() as tmp:
+        ctx.upload(s.pr[order], s.pv[order], np.zeros(n, np.uint16), s.ids[order], a=np.zeros((n, 3)),
+                   dti=np.zeros(n, np.uint32))
+        ctx.set_counts(n, n)
+        path = Path(tmp) / "tracks.bin"
+        rec = ctx.track_records(s.m, s.r, s.v)
+        native.write_track_records(path, 0.0, planets, rec)
+        pack, write = [], []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            rec = ctx.track_records(s.m, s.r, s.v)
+            t1 = time.perf_counter()
+            native.write_track_records(path, 0.0, planets, rec)
+            write.append(time.perf_counter() - t1)
+            pack.append(t1 - t0)
+        size = path.stat().st_size
+    pack_ms, write_ms = 1e3 * float(np.median(pack)), 1e3 * float(np.median(write))
+    return {"particles": n, "record_bytes": TRACK_RECORD_BYTES, "file_bytes": size,
+            "pack_ms": pack_ms, "write_ms": write_ms, "value": n / ((pack_ms + write_ms) / 1e3),
+            "unit": "particles/s", "records": rec}
+
+
+def cpu_track(inputs, records) -> dict:
+    """The reference's own save_binary_track (oracle/_ref) over the same
+    particles, already in id order (its sort_by_id is not timed), to a track
+    file: host clock, median of 3; and, as the checker, the device records
+    against the records of that file (same ids; Cartesian-to-element floats
+    equal or one float32 ulp apart)."""
+    import tempfile
+    from oracle.pyoracle import Oracle
+
+    s = inputs[0]
+    order = np.argsort(s.ids)
+    st = {"pm": s.m, "pr": s.r, "pv": s.v, "pid": np.arange(s.n_bodies, dtype=np.uint32), "r": s.pr[order],
+          "v": s.pv[order], "id": s.ids[order]}
+    o = Oracle(reference=True)
+    times = []
+    with tempfile.TemporaryDirectory() as tmp:
+        path = Path(tmp) / "tracks.bin"
+        for _ in range(3):
+            t = time.perf_counter()
+            o.track_write(path, 0.0, st)
+            times.append(time.perf_counter() - t)
+        data = path.read_bytes()
+    head = 8 + 8 + (s.n_bodies - 1) * TRACK_RECORD_BYTES + 8
+    ref = np.frombuffer(data[head:], dtype=np.dtype([("id", np.uint32), ("x", np.float32, 6)]))
+    same_ids = bool(len(ref) == len(records) and np.array_equal(ref["id"], records["id"]))
+    out = {"value": len(ref) / float(np.median(times)), "unit": "particles/s", "cores": 1, "kind": "reference",
+           "ms": 1e3 * float(np.median(times)), "ids_equal": same_ids}
+    if same_ids:
+        ulps = np.abs(ref["x"].view(np.int32).astype(np.int64) - records["x"].view(np.int32).astype(np.int64))
+        out["floats_equal"] = float(np.mean(ulps == 0))
+        out["max_ulps"] = int(ulps.max())
+    return out
+
+
 def cpu_baseline(inputs, sample_particles: int, runs: int = 3, threads: int = 0):
     """Reference CPU path (oracle/_ref: the reference's own src/wh.cpp built
     from its sources) or, where that library is absent, the oracle
@@ -434,6 +516,7 @@ def main(argv=None):
     ap.add_argument("--batch-launches", type=int, default=BATCH_LAUNCHES)
     ap.add_argument("--no-encounter", action="store_true", help="skip the encounter hand-off workload")
     ap.add_argument("--encounter-particles", type=int, default=ENC_PARTICLES)
+    ap.add_argument("--no-track", action="store_true", help="skip the track-output workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -529,18 +612,27 @@ def main(argv=None):
         line["precision_vs_fp64"] = {
             "after_steps": (args.warmup + args.steps) * TBSIZE,
             **{f: element_differences(inputs[0].m[0], finals["fp64"], finals[f]) for f in finals if f != "fp64"}}
-    if args.oracle_sample > 0:
-        n_s = min(args.oracle_sample, args.particles)
-        ref = oracle_sample_run(inputs, n_s, args.warmup + args.steps)
-        line["precision_vs_oracle"] = {
-            "after_steps": (args.warmup + args.steps) * TBSIZE, "sample": n_s,
-            **{f: element_differences(inputs[0].m[0], ref, subset_by_id(finals[f], ref["id"])) for f in finals}}
     if not args.no_batch:
         line["batch"] = run_batch(forms, args.batch_systems, args.batch_launches, args.seed + 2)
     if not args.no_encounter:
         line["encounter"] = run_encounter(forms, args.encounter_particles, ENC_BLOCKS, args.seed + 3)
+    records = None
+    if not args.no_track:
+        line["track"] = run_track(inputs)
+        records = line["track"].pop("records")
     if rank == 0 and not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(inputs, args.cpu_sample, threads=args.cpu_threads)
+        # the CPU leg: the reference's CPU path timed on the host and, as the
+        # checker, the oracle over a particle sample of the timed run
+        cpu = cpu_baseline(inputs, args.cpu_sample, threads=args.cpu_threads)
+        if args.oracle_sample > 0:
+            n_s = min(args.oracle_sample, args.particles)
+            ref = oracle_sample_run(inputs, n_s, args.warmup + args.steps)
+            cpu["precision_vs_oracle"] = {
+                "after_steps": (args.warmup + args.steps) * TBSIZE, "sample": n_s,
+                **{f: element_differences(inputs[0].m[0], ref, subset_by_id(finals[f], ref["id"])) for f in finals}}
+        if records is not None and cpu["kind"] == "reference":
+            cpu["track"] = cpu_track(inputs, records)
+        line["cpu_baseline"] = cpu
     if rank == 0:
         print(json.dumps(line), flush=True)
     return 0

@@ -136,26 +136,34 @@ __global__ __launch_bounds__(kThreads) void tile_sums_kernel(const uint32_t* __r
     }
 }
 
-// Exclusive scan of the tile sums in one workgroup (serial per thread over
-// its slice, then across threads).
+// Exclusive scan of the tile sums in one workgroup of 1024 lanes: each lane
+// sums its slice serially, the lanes' sums are scanned by wave shuffles
+// (inclusive within each wave, then over the 16 wave totals), and each lane
+// writes its slice's exclusive prefixes.
 __global__ __launch_bounds__(1024) void scan_tiles_kernel(uint32_t* sums, uint32_t n_tiles) {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t wave_tot[16];
     const uint32_t per = (n_tiles + 1023) / 1024;
     const uint32_t b = threadIdx.x * per;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t s = 0;
     for (uint32_t k = b; k < b + per && k < n_tiles; k++) s += sums[k];
-    part[threadIdx.x] = s;
+    uint32_t inc = s;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) wave_tot[w] = inc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (uint32_t t = 0; t < 1024; t++) {
-            const uint32_t x = part[t];
-            part[t] = run;
-            run += x;
+    if (w == 0) {
+        uint32_t t = lane < 16 ? wave_tot[lane] : 0;
+        for (uint32_t off = 1; off < 16; off <<= 1) {
+            const uint32_t y = __shfl_up(t, off, 64);
+            if (lane >= off) t += y;
         }
+        if (lane < 16) wave_tot[lane] = t;
     }
     __syncthreads();
-    uint32_t run = part[threadIdx.x];
+    uint32_t run = inc - s + (w ? wave_tot[w - 1] : 0);
     for (uint32_t k = b; k < b + per && k < n_tiles; k++) {
         const uint32_t x = sums[k];
         sums[k] = run;
@@ -222,25 +230,28 @@ __global__ __launch_bounds__(kThreads) void pack_track_kernel(ParticleSet s, uin
     }
 }
 
-// max over id[0, n) into *out by one workgroup.
-__global__ __launch_bounds__(1024) void max_id_kernel(const uint32_t* __restrict__ id, uint32_t n, uint32_t* out) {
-    __shared__ uint32_t part[16];
+// max over id[0, n) folded into *out (cleared beforehand): each workgroup
+// reduces its grid-stride share, then one atomic max per workgroup.
+__global__ __launch_bounds__(kThreads) void max_id_kernel(const uint32_t* __restrict__ id, uint32_t n, uint32_t* out) {
+    __shared__ uint32_t part[kThreads / 64];
     uint32_t m = 0;
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) m = max(m, id[k]);
+    for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < n; k += gridDim.x * kThreads) m = max(m, id[k]);
     for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_down(m, off, 64));
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = 0;
-        for (uint32_t w = 0; w < 16; w++) t = max(t, part[w]);
-        out[0] = t;
+        for (uint32_t w = 0; w < kThreads / 64; w++) t = max(t, part[w]);
+        atomicMax(out, t);
     }
 }
 
 }  // namespace
 
 hipError_t launch_max_id(const uint32_t* id, uint32_t n, uint32_t* out, hipStream_t stream) {
-    hipLaunchKernelGGL(max_id_kernel, dim3(1), dim3(1024), 0, stream, id, n, out);
+    const uint32_t g = grid_for(n);
+    hipLaunchKernelGGL(clear_u32_kernel, dim3(1), dim3(kThreads), 0, stream, out, 1u);
+    hipLaunchKernelGGL(max_id_kernel, dim3(g < 1 ? 1 : g > 256 ? 256 : g), dim3(kThreads), 0, stream, id, n, out);
     return hipGetLastError();
 }
 

@@ -1,11 +1,12 @@
 """bench.py's contract pieces that run without a GPU: the algorithmic FLOP
 count it prices the roofline with, the refusal of more than one GPU, the
-per-form launch defaults and the CPU baseline's single-thread and all-cores
-legs."""
+per-form launch defaults, the CPU baseline's single-thread and all-cores
+legs, and its track-record check."""
 import argparse
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 from tests.conftest import ROOT
@@ -48,3 +49,25 @@ def test_cpu_baseline_slices_cover_the_swarm(bench):
     assert ac["cores"] == 3 and ac["value"] > 0
     assert f"({96 * bench.TBSIZE} particle-steps" in ac["sample"]
     assert set(r["host"]) == {"cpu_model", "logical_cpus", "libc", "compiler"}
+
+
+def test_cpu_track_check(bench, tmp_path):
+    # records from the host track writer (byte-equal to the reference's) pass
+    # the check exactly; one float moved by one ulp shows up as such
+    from glisser_amd import native
+    from oracle.pyoracle import reference_available
+    if not reference_available():
+        pytest.skip("reference build absent")
+    inputs = bench.make_inputs(300, 1, 4)
+    s = inputs[0]
+    o = np.argsort(s.ids)
+    st = {"pm": s.m, "pr": s.r, "pv": s.v, "pid": np.arange(s.n_bodies, dtype=np.uint32), "r": s.pr[o],
+          "v": s.pv[o], "id": s.ids[o]}
+    native.write_track(tmp_path / "h.trk", 0.0, st)
+    head = 8 + 8 + (s.n_bodies - 1) * bench.TRACK_RECORD_BYTES + 8
+    rec = np.frombuffer((tmp_path / "h.trk").read_bytes()[head:], dtype=native.TRACK_DTYPE).copy()
+    r = bench.cpu_track(inputs, rec)
+    assert r["ids_equal"] and r["floats_equal"] == 1.0 and r["max_ulps"] == 0 and r["value"] > 0
+    rec["x"][7, 2] = np.nextafter(rec["x"][7, 2], np.float32(np.inf))
+    r = bench.cpu_track(inputs, rec)
+    assert r["max_ulps"] == 1 and r["floats_equal"] == 1.0 - 1.0 / rec["x"].size

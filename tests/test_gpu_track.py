@@ -94,3 +94,31 @@ def test_sparse_ids_and_errors(native):
             ctx.track_records(s.m, s.r, s.v)
     finally:
         ctx.close()
+
+
+def test_rank_scan_many_tiles(native):
+    # 400,000 particles with ids drawn from [0, 1.5M): 1,465 tiles of the id
+    # rank scan, more than its 1,024 lanes, so lanes carry two tiles each;
+    # 1 in 5 particles flagged (not in the records)
+    rng = np.random.default_rng(93)
+    n = 400_000
+    pr = rng.normal(0.0, 30.0, (n, 3))
+    pv = rng.normal(0.0, 3e-3, (n, 3))
+    ids = rng.choice(1_500_000, n, replace=False).astype(np.uint32)
+    flags = np.where(rng.random(n) < 0.2, 0x80, 0).astype(np.uint16)
+    m = np.array([2.959122e-4, 2.8e-7])
+    r_pl = np.array([[0.0, 0.0, 0.0], [5.2, 0.0, 0.0]])
+    v_pl = np.array([[0.0, 0.0, 0.0], [0.0, 7.5e-3, 0.0]])
+    ctx = native.Context(n, 2, 16)
+    try:
+        ctx.upload(pr, pv, flags, ids, a=np.zeros((n, 3)), dti=np.zeros(n, np.uint32))
+        ctx.set_counts(n, n)
+        ctx.partition()
+        rec = ctx.track_records(m, r_pl, v_pl, False, False)
+    finally:
+        ctx.close()
+    alive = flags == 0
+    order = np.argsort(ids[alive])
+    np.testing.assert_array_equal(rec["id"], ids[alive][order])
+    np.testing.assert_array_equal(rec["x"][:, :3], pr[alive][order].astype(np.float32))
+    np.testing.assert_array_equal(rec["x"][:, 3:], pv[alive][order].astype(np.float32))
