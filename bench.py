@@ -22,8 +22,9 @@ it, with their orbital-element differences from the fp64 form at the end
 of the run.
 
 Beside the headline: a batched launch of 64 systems (9.45M particles), a
-1M-particle three-body run that hands encounter records to the host, and
-the track output of the 200k swarm (orbital elements formed on the device
+1M-particle three-body run that hands encounter records to the host, the
+reference's own CPU-runnable case end to end (1,000 particles, planets
+integrated on the host, 50 blocks), and the track output of the 200k swarm (orbital elements formed on the device
 as the reference's 28-byte records, then written to a track file).  The
 CPU leg (skipped by --no-cpu) times the reference's CPU path on the host,
 and runs the oracle over a particle sample as the checker.
@@ -335,6 +336,99 @@ def run_encounter(forms, n_particles: int, n_blocks: int, seed: int) -> dict:
     return out
 
 
+CONFIG0_PARTICLES = 1000
+CONFIG0_BLOCKS = 50
+
+
+def run_config0(forms, seed: int = 1) -> dict:
+    """The reference's own CPU-runnable case end to end: Sun + 4 giants
+    integrated on the host (the Jacobi WH planet producer, overlapped with
+    the kernel), 1,000 particles a ~ U[60, 65] au, dt = 180 d, 50 blocks of
+    2048 steps (102,400 steps), cull radius 0.5 au, kill mode, through
+    glx::Executor: one call of 50 blocks per form, host clock; particle-steps
+    from the death times.  Returns the fp64 form's final state under
+    "final"."""
+    from glisser_amd import native, systems
+
+    s = systems.outer_solar_system(CONFIG0_PARTICLES, seed=seed)
+    t_e = CONFIG0_BLOCKS * TBSIZE * DT
+    out = {"particles": s.n, "blocks": CONFIG0_BLOCKS, "tbsize": TBSIZE, "dt_days": DT, "forms": {}}
+    for form in forms:
+        with native.Executor(s.m, s.r, s.v, s.pr, s.pv, s.ids, dt=DT, tbsize=TBSIZE, outer_radius=OUTER,
+                             form=form) as ex:
+            t0 = time.perf_counter()
+            ex.step(CONFIG0_BLOCKS)
+            el = time.perf_counter() - t0
+            d = ex.download()
+        death = np.where(np.isnan(d["deathtime"]), t_e, np.minimum(d["deathtime"], t_e))
+        psteps = int(np.rint(death / DT).sum())
+        out["forms"][form] = {"value": psteps / el, "unit": "particle-steps/s", "ms": el * 1e3,
+                              "particle_steps": psteps, "alive_end": int(d["n_alive"])}
+        if form == "fp64":
+            out["final"] = d
+    return out
+
+
+def cpu_config0(gpu_final=None, seed: int = 1) -> dict:
+    """The reference's CPU path on the same case end to end, one thread, host
+    clock (oracle/_ref: its planet producer, particle time-blocks and stable
+    partition, encounters killed, as targets/main.cpp drives them); as the
+    checker, the GPU's fp64 final state against it (survivor count,
+    relative position error by id)."""
+    from glisser_amd import systems
+    from oracle.pyoracle import Oracle, reference_available
+
+    s = systems.outer_solar_system(CONFIG0_PARTICLES, seed=seed)
+    kind = "reference" if reference_available() else "port"
+    o = Oracle(reference=(kind == "reference"))
+
+    def kill(f):
+        f = f.copy()
+        enc = (f & 0x01) == 0x01
+        f[enc] = (f[enc] & 0xFF00) | 0x80
+        return f
+
+    t0 = time.perf_counter()
+    pl = o.planets(s.m, s.r, s.v, cull_radius=CULL, rh_factor=0.0, outer=OUTER)
+    n = s.n
+    r, v, ids = s.pr.copy(), s.pv.copy(), s.ids.copy()
+    a, flags = o.particle_accels(s.m, pl.state()[2], OUTER, s.r[1:], pl.h0_0, r, np.zeros(n, np.uint16), ids)
+    flags = kill(flags)
+    n_alive, perm = o.stable_partition(flags, 0, n, 0)
+    r, v, a, flags, ids = r[perm], v[perm], a[perm], flags[perm], ids[perm]
+    dti = np.zeros(n, np.uint32)  # indexed by id, as in the reference
+    t, psteps = 0.0, 0
+    for _ in range(CONFIG0_BLOCKS):
+        pl.recalculate_rh()
+        rh = pl.state()[2]
+        rl, _vl, h0 = pl.block(TBSIZE, DT)
+        res = o.particle_block(s.m, rh, OUTER, rl, h0, t, DT, r[:n_alive], v[:n_alive], a[:n_alive],
+                               flags[:n_alive], ids[:n_alive], dti)
+        if res["rc"] != 0:
+            raise RuntimeError(f"reference CPU path failed: {o.err()}")
+        dti = res["dti"]
+        psteps += int(dti[ids[:n_alive]].astype(np.int64).sum())
+        sl = slice(0, n_alive)
+        r[sl], v[sl], a[sl], flags[sl] = res["r"], res["v"], res["a"], kill(res["flags"])
+        split, perm = o.stable_partition(flags, 0, n_alive, 0)
+        r[sl], v[sl], a[sl], flags[sl], ids[sl] = r[sl][perm], v[sl][perm], a[sl][perm], flags[sl][perm], ids[sl][perm]
+        n_alive = split
+        t += DT * TBSIZE
+    el = time.perf_counter() - t0
+    out = {"value": psteps / el, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+           "sample": f"{n} particles + Sun and 4 giants integrated on the host, {CONFIG0_BLOCKS} blocks of "
+                     f"{TBSIZE} steps ({psteps} particle-steps, {el:.1f} s, single thread)",
+           "particle_steps": psteps, "alive_end": int(n_alive)}
+    if gpu_final is not None:
+        og, oc = np.argsort(gpu_final["id"]), np.argsort(ids)
+        same = bool(np.array_equal(gpu_final["id"][og], ids[oc]) and gpu_final["n_alive"] == n_alive)
+        out["fp64_vs_cpu"] = {"same_survivors": same}
+        if same:
+            err = (np.linalg.norm(gpu_final["r"][og] - r[oc], axis=1) / np.linalg.norm(r[oc], axis=1))
+            out["fp64_vs_cpu"].update({"r_rel_max": float(err.max()), "r_rel_median": float(np.median(err))})
+    return out
+
+
 def run_track(inputs, repeats: int = 5, seed: int = 5) -> dict:
     """Track output of the whole swarm, the reference's particle-history pass
     (targets/main.cpp:148-156 sort_by_id, src/data.cpp:770-812
@@ -517,6 +611,8 @@ def main(argv=None):
     ap.add_argument("--no-encounter", action="store_true", help="skip the encounter hand-off workload")
     ap.add_argument("--encounter-particles", type=int, default=ENC_PARTICLES)
     ap.add_argument("--no-track", action="store_true", help="skip the track-output workload")
+    ap.add_argument("--no-config0", action="store_true",
+                    help="skip the end-to-end 1,000-particle case with host-integrated planets")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the known-size copy kernel (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -616,6 +712,10 @@ def main(argv=None):
         line["batch"] = run_batch(forms, args.batch_systems, args.batch_launches, args.seed + 2)
     if not args.no_encounter:
         line["encounter"] = run_encounter(forms, args.encounter_particles, ENC_BLOCKS, args.seed + 3)
+    c0_final = None
+    if not args.no_config0:
+        line["config0"] = run_config0(forms)
+        c0_final = line["config0"].pop("final", None)
     records = None
     if not args.no_track:
         line["track"] = run_track(inputs)
@@ -632,6 +732,8 @@ def main(argv=None):
                 **{f: element_differences(inputs[0].m[0], ref, subset_by_id(finals[f], ref["id"])) for f in finals}}
         if records is not None and cpu["kind"] == "reference":
             cpu["track"] = cpu_track(inputs, records)
+        if not args.no_config0:
+            cpu["config0"] = cpu_config0(c0_final)
         line["cpu_baseline"] = cpu
     if rank == 0:
         print(json.dumps(line), flush=True)

@@ -1,7 +1,8 @@
 """bench.py's contract pieces that run without a GPU: the algorithmic FLOP
 count it prices the roofline with, the refusal of more than one GPU, the
 per-form launch defaults, the CPU baseline's single-thread and all-cores
-legs, and its track-record check."""
+legs, its track-record check and its end-to-end run of the reference's own
+case."""
 import argparse
 import subprocess
 import sys
@@ -71,3 +72,19 @@ def test_cpu_track_check(bench, tmp_path):
     rec["x"][7, 2] = np.nextafter(rec["x"][7, 2], np.float32(np.inf))
     r = bench.cpu_track(inputs, rec)
     assert r["max_ulps"] == 1 and r["floats_equal"] == 1.0 - 1.0 / rec["x"].size
+
+
+def test_cpu_config0_loop(bench, oracle, monkeypatch):
+    # the end-to-end CPU leg (2 of its 50 blocks here) is the same loop as
+    # the parity tests' CPU path: same survivors, same positions, exact
+    # particle-step count, and its checker reads that as zero error
+    from glisser_amd import systems
+    from tests.helpers import oracle_run, planet_blocks
+    monkeypatch.setattr(bench, "CONFIG0_BLOCKS", 2)
+    s = systems.outer_solar_system(bench.CONFIG0_PARTICLES, seed=1)
+    h00, rh0, blocks = planet_blocks(oracle, s, 2, bench.TBSIZE, bench.DT)
+    ref = oracle_run(oracle, s, h00, rh0, blocks, bench.DT, bench.OUTER)
+    r = bench.cpu_config0({"id": ref["id"], "r": ref["r"], "n_alive": ref["n_alive"]})
+    assert r["particle_steps"] == bench.CONFIG0_PARTICLES * 2 * bench.TBSIZE
+    assert r["alive_end"] == ref["n_alive"] == bench.CONFIG0_PARTICLES
+    assert r["fp64_vs_cpu"] == {"same_survivors": True, "r_rel_max": 0.0, "r_rel_median": 0.0}
