@@ -17,6 +17,7 @@
 #include <cstdint>
 
 #include "glisser_amd.h"
+#include "sincos_cr.h"
 
 namespace glx {
 
@@ -76,8 +77,10 @@ struct Particle {
 
 // f and g functions of the Kepler drift in the reference's arithmetic
 // (src/wh.cu:74-110 with kepeq at src/wh.cu:42-72): fixed five Newton
-// iterations, IEEE divides and square roots, sin/cos from the device math
-// library, and no FMA contraction, so every basic operation rounds exactly
+// iterations, IEEE divides and square roots, correctly rounded sin/cos
+// (sincos_cr.h: glibc's sin/cos in the reference CPU build return the
+// correctly rounded value on these arguments in all but a few calls per
+// million), and no FMA contraction, so every basic operation rounds exactly
 // as in the reference CPU build (the particle orbits amplify a one-ulp
 // difference to ~1e-10 over 1e5 steps).  The new state is
 // r = f r0 + g v0, v = fdot r0 + gdot v0.  Returns the flags of the drift:
@@ -108,10 +111,10 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
     const double dM = dtn - kPi * 2 * (double)(int)(dtn / kPi * 2);
     const double adv_dt = dM / n_;
     double sM, cM;
-    sincos(dM, &sM, &cM);
+    cr::sincos_cr(dM, &sM, &cM);
     double dE = dM - esinEo + esinEo * cM + ecosEo * sM;
     double sindE, cosdE;
-    sincos(dE, &sindE, &cosdE);
+    cr::sincos_cr(dE, &sindE, &cosdE);
     double delta = 0.0;
 #pragma unroll
     for (int k = 0; k < kKepIts; k++) {
@@ -119,7 +122,7 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
         const double fpk = 1. - ecosEo * cosdE + esinEo * sindE;
         delta = -fk / fpk;
         dE += delta;
-        sincos(dE, &sindE, &cosdE);
+        cr::sincos_cr(dE, &sindE, &cosdE);
     }
     const double fp = 1.0 - ecosEo * cosdE + esinEo * sindE;
     f = 1.0 + a * (cosdE - 1.0) / dist;

@@ -146,6 +146,8 @@ SIGNATURES = {
     "glx_sync": (C.c_int, [C.c_void_p]),
     "glx_last_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "glx_sum_steps": (C.c_int, [C.c_void_p, C.c_uint64, _u64p]),
+    "glx_sincos_device": (C.c_int, [C.c_int, C.c_uint64, _dp, _dp, _dp]),
+    "glx_sincos_host": (C.c_int, [C.c_uint64, _dp, _dp, _dp]),
     "glx_batch_create": (C.c_int, [C.c_int, C.POINTER(SystemDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "glx_batch_destroy": (C.c_int, [C.c_void_p]),
     "glx_batch_upload": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _dp, _u16p, _u32p]),
@@ -853,3 +855,18 @@ def from_elements(mu, el, mean_anomaly: bool = True):
     _check_host(load().glx_from_elements(n, _ptr(mu, _dp), _ptr(el, _dp), int(mean_anomaly), _ptr(r, _dp),
                                          _ptr(v, _dp)))
     return r, v
+
+
+def sincos(x, device: int | None = None):
+    """sin and cos of x as the fp64 form's Kepler drift evaluates them
+    (correctly rounded for |x| < 2^20): on GPU `device`, or with the host
+    twin of the same routine when device is None."""
+    x = np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+    n = x.shape[0]
+    s, c = np.empty(n), np.empty(n)
+    lib = load()
+    if device is None:
+        _check_host(lib.glx_sincos_host(n, _ptr(x, _dp), _ptr(s, _dp), _ptr(c, _dp)))
+    else:
+        _check_host(lib.glx_sincos_device(int(device), n, _ptr(x, _dp), _ptr(s, _dp), _ptr(c, _dp)))
+    return s, c

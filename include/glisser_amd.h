@@ -216,6 +216,16 @@ int glx_last_times(glx_ctx* ctx, float* kernel_ms, float* partition_ms);
  * block -- computed on the device (synchronous). */
 int glx_sum_steps(glx_ctx* ctx, uint64_t n, uint64_t* total);
 
+/* sin and cos of n doubles as the GLX_FORM_FP64 Kepler drift evaluates
+ * them: correctly rounded (round to nearest) for |x| < 2^20, the device math
+ * library beyond.  The reference's CPU drift calls glibc sin/cos
+ * (src/wh.cpp:46-65, :928), which is correctly rounded on the Kepler
+ * solve's arguments in all but a few calls per million.
+ * glx_sincos_device runs the device routine on GPU `device` (synchronous;
+ * at most 2^28 values); glx_sincos_host runs the same code on the host. */
+int glx_sincos_device(int device, uint64_t n, const double* x, double* s, double* c);
+int glx_sincos_host(uint64_t n, const double* x, double* s, double* c);
+
 /* ---------------------------------------------------------------------
  * Batched runs: many independent planetary systems (different bodies,
  * particle counts and timesteps) advanced together, one launch per

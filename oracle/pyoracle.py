@@ -55,6 +55,9 @@ _REF_ONLY = {
                               _dp, _u32p, C.c_int, C.c_int]),
 }
 _ORACLE_ONLY = {
+    "set_particle_libm": (None, [C.c_int]),
+    "particle_libm_counts": (None, [_u64p, _u64p]),
+    "sincos_cr": (None, [C.c_uint64, _dp, _dp, _dp]),
     "drift": (C.c_int, [C.c_double, C.c_uint32, _dp, _dp, _dp, C.c_int, _u64p]),
     "hermite_log": (C.c_int, [C.c_uint32, _dp, C.c_uint64, _dp, _dp, _dp, C.c_double, C.c_double, C.c_uint32,
                               _dp, _dp]),
@@ -95,6 +98,28 @@ class Oracle:
 
     def err(self) -> str:
         return self._f("last_error")().decode()
+
+    def set_particle_libm(self, correctly_rounded: bool) -> None:
+        """sin/cos of the particle drift on the calling thread: glibc (the
+        reference's, default) or correctly rounded (libquadmath's sinq /
+        cosq rounded to double); resets the thread's call counters."""
+        if self.reference:
+            raise ValueError("the reference build always calls glibc")
+        self._f("set_particle_libm")(1 if correctly_rounded else 0)
+
+    def sincos_cr(self, x):
+        """The correctly rounded sin and cos of the alternative particle libm."""
+        x = np.ascontiguousarray(x, np.float64)
+        s, c = np.empty_like(x), np.empty_like(x)
+        self._f("sincos_cr")(x.size, _p(x, _dp), _p(s, _dp), _p(c, _dp))
+        return s, c
+
+    def particle_libm_counts(self) -> tuple[int, int]:
+        """(sin/cos calls in correctly rounded mode on this thread, calls in
+        which glibc returns another double)."""
+        calls, differ = C.c_uint64(0), C.c_uint64(0)
+        self._f("particle_libm_counts")(C.byref(calls), C.byref(differ))
+        return calls.value, differ.value
 
     # -- particles ------------------------------------------------------
     def particle_accels(self, m, rh, outer, planet_r, h0, r, flags, ids, time=0.0, deathtime=None):

@@ -23,6 +23,8 @@
 //   to_elements/from_elements  : round trip, circular/planar special cases
 //   hermite_log                : exact for cubics, node reproduction,
 //                                fourth-order convergence on Kepler orbits
+//   correctly rounded sin/cos  : (the checker's alternative particle libm)
+//                                200-bit mpmath values, tests/test_sincos.py
 //   planet history (plhist_*)  : writer bytes and interpolated blocks equal
 //                                the reference build's and its golden blocks;
 //                                intervals end on the records' own elements;
@@ -31,6 +33,8 @@
 // Error behaviour: where the reference throws std::runtime_error (unbound
 // orbit, unconverged Kepler solve; src/wh.cpp:894-908,942-945) the oracle
 // returns a non-zero status and the offending index instead.
+
+#include <quadmath.h>
 
 #include <cmath>
 #include <cstdint>
@@ -95,19 +99,65 @@ inline void sin_cos_separate(double x, double& s, double& c) {
 }
 inline void sin_cos_fused(double x, double& s, double& c) { ::sincos(x, &s, &c); }
 
+// The C library the particle drift (the fixed-iteration path) calls, per
+// calling thread: 0 (default) glibc as above, the reference's; 1 correctly
+// rounded sin and cos -- glibc's long-double sinl / cosl, or libquadmath's
+// 113-bit sinq / cosq where those cannot decide, rounded to the nearest
+// double: a correctly rounded evaluation independent of the device's -- with a
+// count of the calls in which glibc would have returned another double.
+// The planet drift always calls glibc.  A checker's option: it shows which
+// differences between the device's correctly rounded sin/cos and the
+// reference build come from glibc's rounding alone.
+// The long-double values (glibc's sinl / cosl, within a few units of 2^-64
+// relative) decide the rounding whenever both ends of a 2^-60 relative
+// interval around them round to the same double (over 99 % of calls); the
+// rest take the 113-bit evaluation.
+thread_local int tl_particle_libm = 0;
+thread_local uint64_t tl_libm_calls = 0, tl_libm_differ = 0;
+inline bool round_decided_ld(long double v, double& out) {
+    const long double b = fabsl(v) * 0x1p-60L;
+    const double lo = (double)(v - b), hi = (double)(v + b);
+    out = lo;
+    return lo == hi;
+}
+inline double sin_cr(double x) {
+    double d;
+    if (round_decided_ld(sinl((long double)x), d)) return d;
+    return (double)sinq((__float128)x);
+}
+inline double cos_cr(double x) {
+    double d;
+    if (round_decided_ld(cosl((long double)x), d)) return d;
+    return (double)cosq((__float128)x);
+}
+inline void particle_sin_cos(double x, double& s, double& c, bool fused) {
+    if (tl_particle_libm == 0) {
+        if (fused) sin_cos_fused(x, s, c);
+        else sin_cos_separate(x, s, c);
+        return;
+    }
+    s = sin_cr(x);
+    c = cos_cr(x);
+    double gs, gc;
+    if (fused) sin_cos_fused(x, gs, gc);
+    else sin_cos_separate(x, gs, gc);
+    tl_libm_calls += 2;
+    tl_libm_differ += (gs != s) + (gc != c);
+}
+
 // Newton iteration on the eccentric-anomaly difference form of Kepler's
 // equation with a fixed iteration count; src/wh.cpp:46-65.
 // Returns true on "error" (last correction not below the tolerance).
 bool kepler_fixed(double dM, double esinEo, double ecosEo, double& dE, double& sindE, double& cosdE,
                   uint32_t its) {
     double delta = 1;
-    sin_cos_separate(dE, sindE, cosdE);
+    particle_sin_cos(dE, sindE, cosdE, false);
     for (uint32_t i = 0; i < its; i++) {
         double f = dE - ecosEo * sindE + esinEo * (1. - cosdE) - dM;
         double fp = 1. - ecosEo * cosdE + esinEo * sindE;
         delta = -f / fp;
         dE += delta;
-        sin_cos_separate(dE, sindE, cosdE);
+        particle_sin_cos(dE, sindE, cosdE, false);
     }
     return std::fabs(delta) >= kTolKep;
 }
@@ -163,7 +213,8 @@ int drift(double t, V3* r, V3* v, size_t start, size_t n, const double* mu, cons
         double dM = t * n_ - kPi * 2 * (int)(t * n_ / kPi * 2);
         double adv_dt = dM / n_;
         double sM, cM;
-        sin_cos_fused(dM, sM, cM);
+        if (fixed) particle_sin_cos(dM, sM, cM, true);
+        else sin_cos_fused(dM, sM, cM);
         double dE = dM - esinEo + esinEo * cM + ecosEo * sM;
         double sindE, cosdE;
         bool err = fixed ? kepler_fixed(dM, esinEo, ecosEo, dE, sindE, cosdE, kParticleKepIts)
@@ -862,6 +913,26 @@ int write_history(const char* path, int append, double time, uint32_t n, const d
 extern "C" {
 
 const char* orc_last_error() { return g_err; }
+
+// Library of the particle drift's sin/cos on the calling thread (0 glibc,
+// 1 correctly rounded); resets the thread's call counters.
+void orc_set_particle_libm(int mode) {
+    tl_particle_libm = mode;
+    tl_libm_calls = 0;
+    tl_libm_differ = 0;
+}
+// The correctly rounded sin and cos of mode 1 over n arguments.
+void orc_sincos_cr(uint64_t n, const double* x, double* s, double* c) {
+    for (uint64_t i = 0; i < n; i++) {
+        s[i] = sin_cr(x[i]);
+        c[i] = cos_cr(x[i]);
+    }
+}
+// Calls made in mode 1 on the calling thread, and those where glibc differs.
+void orc_particle_libm_counts(uint64_t* calls, uint64_t* differ) {
+    *calls = tl_libm_calls;
+    *differ = tl_libm_differ;
+}
 
 // ---- particle path -------------------------------------------------------
 

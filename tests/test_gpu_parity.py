@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 from tests.conftest import rel_err
-from tests.helpers import compare_flags, gpu_run, kill_encounters, oracle_run, planet_blocks
+from tests.helpers import (compare_flags, gpu_run, kill_encounters, nudge_response, oracle_run, oracle_run_parallel,
+                           planet_blocks, unexplained_flag_mismatches)
 
 pytestmark = pytest.mark.gpu
 
@@ -67,37 +68,32 @@ def test_empty_launch(native):
         assert ctx.sum_steps(0) == 0
 
 
-def _regular_mask(oracle, s, h0_0, rh0, blocks, cpu):
-    """Particles whose CPU trajectory is insensitive to a 1-ulp nudge of the
-    initial position (relative divergence < 1e-12): the 'non-chaotic' set
-    the 1e-10 bar applies to."""
-    import copy
-    s2 = copy.deepcopy(s)
-    s2.pr = np.nextafter(s.pr, np.inf)
-    cpu2 = oracle_run(oracle, s2, h0_0, rh0, blocks, DT, OUTER)
-    oc, o2 = np.argsort(cpu["id"]), np.argsort(cpu2["id"])
-    div = rel_err(cpu2["r"][o2], cpu["r"][oc])
-    return div < 1e-12
-
-
 def test_1e5_steps_regular_particles(native, oracle):
-    """49 blocks x 2048 = 100,352 steps: relative 1e-10 on r and v for every
-    regular particle, identical flag sets."""
+    """49 blocks x 2048 = 100,352 steps, fp64 form: bit for bit the CPU path
+    run with correctly rounded sin/cos (the device's sin/cos are; every
+    other operation rounds as the CPU path does), flags identical.  Against
+    the CPU path with glibc (which misrounds a few sin/cos calls per
+    million), relative 1e-10 on r and v for at least 60 of the 64
+    particles and 1e-9 for all."""
     from glisser_amd import systems
     s = systems.outer_solar_system(64, seed=7)
     h0_0, rh0, blocks = planet_blocks(oracle, s, 49, 2048, DT)
     cpu = oracle_run(oracle, s, h0_0, rh0, blocks, DT, OUTER)
+    cr = oracle_run_parallel(oracle, s, h0_0, rh0, blocks, DT, OUTER, correctly_rounded=True)
     with _ctx(native, s.n) as ctx:
         gpu = gpu_run(ctx, s, h0_0, rh0, blocks, DT, OUTER)
-    regular = _regular_mask(oracle, s, h0_0, rh0, blocks, cpu)
     oc, og = np.argsort(cpu["id"]), np.argsort(gpu["id"])
+    np.testing.assert_array_equal(gpu["r"][og], cr["r"])
+    np.testing.assert_array_equal(gpu["v"][og], cr["v"])
+    np.testing.assert_array_equal(gpu["flags"][og], cr["flags"])
+    nudge = nudge_response(oracle, s, h0_0, rh0, blocks, DT, OUTER, cpu)
     er = rel_err(gpu["r"][og], cpu["r"][oc])
     ev = rel_err(gpu["v"][og], cpu["v"][oc])
-    print(f"regular={regular.sum()} max rel err r={er.max():.3e} v={ev.max():.3e} "
-          f"median r={np.median(er):.3e}; regular-only max r={er[regular].max() if regular.any() else 0:.3e}")
-    print("per-particle r err:", np.array2string(np.sort(er), precision=2))
-    assert er[regular].max() < 1e-10
-    assert ev[regular].max() < 1e-10
+    print(f"glibc misrounds {cr['libm_differ']} of {cr['libm_calls']} calls; vs the CPU path: identical "
+          f"{(er == 0).sum()}/64, within 1e-10 {(er < 1e-10).sum()}, r-err max {er.max():.3e} median "
+          f"{np.median(er):.3e}; nudge < 1e-10: {(nudge < 1e-10).sum()}")
+    assert (er < 1e-10).sum() >= 60 and (ev < 1e-10).sum() >= 60
+    assert er.max() < 1e-9 and ev.max() < 1e-9
     low, high = compare_flags(cpu["flags"], gpu["flags"], cpu["id"], gpu["id"])
     assert len(low) == 0 and len(high) == 0
 
@@ -244,15 +240,20 @@ def test_fast_forms_elements_1e5_steps(native, oracle, form):
 
 @pytest.mark.parametrize("form", ["fast", "mixed"])
 def test_fast_forms_encounter_flags(native, oracle, form):
+    """FAST / MIXED flag sets against the CPU path on close encounters: any
+    particle whose flags differ must lie within 1e-9 au of a threshold on
+    the CPU trajectory at the step where either path flagged it."""
     from glisser_amd import systems
     s = systems.three_body(2000, seed=5, a_range=(20.0, 40.0), e_hi=0.6, i_hi_deg=5.0)
     h0_0, rh0, blocks = planet_blocks(oracle, s, 3, 512, 30.0, cull_radius=3.0, rh_factor=3.5)
     cpu = oracle_run(oracle, s, h0_0, rh0, blocks, 30.0, OUTER)
     with _ctx(native, s.n, nb=2, tb=512) as ctx:
         gpu = gpu_run(ctx, s, h0_0, rh0, blocks, 30.0, OUTER, form=form)
-    low, high = compare_flags(cpu["flags"], gpu["flags"], cpu["id"], gpu["id"])
-    print(f"{form}: flagged={int((cpu['flags'] != 0).sum())} mismatched={len(low)}")
-    assert len(low) <= 2 and len(high) <= 2
+    bad, dist, unexplained = unexplained_flag_mismatches(oracle, s, h0_0, rh0, blocks, 30.0, OUTER, cpu, gpu)
+    print(f"{form}: flagged={int((cpu['flags'] != 0).sum())} mismatched={len(bad)} "
+          f"threshold distances (au)={np.array2string(dist, precision=2)}")
+    assert (cpu["flags"] != 0).sum() > 20
+    assert unexplained == [], unexplained
 
 
 @pytest.mark.parametrize("form", ["fp64", "fast"])
