@@ -21,7 +21,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libglisser_amd.so"
 ARCH = os.environ.get("GLX_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["wh_kernels.hip", "compact.hip", "interp.hip", "batch.hip", "track.hip", "sincos.hip", "glx_api.cpp", "batch_api.cpp"]
+HIP_SOURCES = ["wh_kernels.hip", "compact.hip", "interp.hip", "batch.hip", "track.hip", "sincos.hip", "static_part.hip", "glx_api.cpp", "batch_api.cpp"]
 
 # Host-only C++ (planet producer, conversions): g++ with the reference CPU
 # build's arithmetic -- no FP contraction, no fast-math.

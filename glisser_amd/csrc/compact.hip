@@ -297,6 +297,22 @@ hipError_t launch_partition(ParticleSet cur, ParticleSet tmp, const RollbackSet*
     return hipGetLastError();
 }
 
+hipError_t launch_partition_into(ParticleSet src, ParticleSet dst, uint32_t n, int three_way, uint32_t* counts,
+                                 uint32_t* offsets, uint32_t* totals, hipStream_t stream) {
+    const uint32_t tiles = part_tiles(n);
+    if (tiles > 0)
+        hipLaunchKernelGGL(part_count_kernel, dim3(tiles), dim3(kThreads), 0, stream, src.flags, n, three_way, counts,
+                           tiles);
+    // with no tile the scan only writes the two class sizes (zero)
+    hipLaunchKernelGGL(part_scan_kernel, dim3(1), dim3(1024), 0, stream, counts, offsets, totals, tiles);
+    if (tiles > 0) {
+        CopyArgs c{src, dst, {}, {}, 0};
+        hipLaunchKernelGGL(part_scatter_kernel, dim3(tiles), dim3(kThreads), 0, stream, c, n, three_way, offsets,
+                           totals, tiles);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_kill_encounters(uint16_t* flags, uint32_t n, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(kill_encounters_kernel, dim3(grid_for(n)), dim3(kThreads), 0, stream, flags, n);

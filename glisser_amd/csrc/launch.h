@@ -40,6 +40,13 @@ hipError_t launch_partition(ParticleSet cur, ParticleSet tmp, const RollbackSet*
                             uint32_t n, int three_way, uint32_t* counts, uint32_t* offsets, uint32_t* totals,
                             hipStream_t stream);
 
+// The same stable partition from src into dst (distinct arrays, nothing
+// copied back): counts and offsets hold 2 * part_tiles(n) entries each,
+// totals receives the two class sizes.  Writes every scratch entry it
+// reads, so nothing needs clearing first.
+hipError_t launch_partition_into(ParticleSet src, ParticleSet dst, uint32_t n, int three_way, uint32_t* counts,
+                                 uint32_t* offsets, uint32_t* totals, hipStream_t stream);
+
 hipError_t launch_kill_encounters(uint16_t* flags, uint32_t n, hipStream_t stream);
 hipError_t launch_snapshot(ParticleSet cur, RollbackSet rb, uint32_t n, hipStream_t stream);
 // Sum of dti[0, n) into out[0]; out holds 1 + kSumParts entries (the rest

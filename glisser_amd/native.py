@@ -91,6 +91,15 @@ _ENC_DTYPE = np.dtype({"names": ["id", "flags", "step", "t_block", "r0", "v0", "
                        "offsets": [0, 4, 8, 16, 24, 48, 72, 96], "itemsize": 120})
 
 
+class DeviceParticles(C.Structure):
+    """glx_device_particles: caller-owned device arrays of one particle set."""
+    _fields_ = [("r", C.c_void_p * 3), ("v", C.c_void_p * 3), ("a", C.c_void_p * 3), ("flags", C.c_void_p),
+                ("deathtime_index", C.c_void_p), ("id", C.c_void_p)]
+
+
+PARTITION_STATIC_MAX = 1 << 25
+
+
 class ExecParams(C.Structure):
     """glx_exec_params: configuration of the host time-block loop."""
     _fields_ = [
@@ -147,6 +156,8 @@ SIGNATURES = {
     "glx_last_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "glx_sum_steps": (C.c_int, [C.c_void_p, C.c_uint64, _u64p]),
     "glx_sincos_device": (C.c_int, [C.c_int, C.c_uint64, _dp, _dp, _dp]),
+    "glx_partition_static": (C.c_int, [C.POINTER(DeviceParticles), C.POINTER(DeviceParticles), C.c_uint64, C.c_int,
+                                       C.c_void_p, C.c_void_p]),
     "glx_sincos_host": (C.c_int, [C.c_uint64, _dp, _dp, _dp]),
     "glx_batch_create": (C.c_int, [C.c_int, C.POINTER(SystemDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "glx_batch_destroy": (C.c_int, [C.c_void_p]),

@@ -183,6 +183,35 @@ int glx_accel_range(glx_ctx* ctx, uint64_t begin, uint64_t n, uint32_t n_bodies,
  * Replaces KillEncounterKernel (src/executor.cu:63-78,656-660). */
 int glx_kill_encounters(glx_ctx* ctx);
 
+/* Device arrays of one particle set (structure of arrays: x, y, z
+ * components), owned by the caller. */
+typedef struct glx_device_particles {
+    double* r[3];
+    double* v[3];
+    double* a[3];
+    uint16_t* flags;
+    uint32_t* deathtime_index;
+    uint32_t* id;
+} glx_device_particles;
+
+/* Most particles one glx_partition_static call takes. */
+#define GLX_PARTITION_STATIC_MAX (1u << 25)
+
+/* glx_partition without any allocation, for callers that cannot allocate
+ * device or host memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Stable partition of the n <= GLX_PARTITION_STATIC_MAX particles of `in`
+ * into `out` (device arrays, distinct from in's) on the current device,
+ * async on `stream` (a hipStream_t; NULL: the null stream): the same
+ * classes and order as glx_partition (mode as there); class_sizes (device
+ * memory, 2 x u32) receives the number of unflagged particles and of the
+ * encounter class (THREE_WAY; else 0).  The tile counts live in a fixed
+ * 512 KiB device array per GPU, kept until the process ends: calls that
+ * overlap on one GPU race for it and must be ordered by the caller (one
+ * stream, or events).  GLX_EINVAL for larger n. */
+int glx_partition_static(const glx_device_particles* in, const glx_device_particles* out, uint64_t n, int mode,
+                         uint32_t* class_sizes, void* stream);
+
 /* Stable partition of [0, n_alive) moving every particle array (and the
  * rollback copy when kept); synchronous.  *n_unflagged receives the new
  * n_alive; *n_encounter the size of the encounter class (THREE_WAY) or 0.
