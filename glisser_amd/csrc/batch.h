@@ -33,6 +33,8 @@ struct SystemDev {
     double sqrt_mu;
     double inv_sqrt_mu;
     double outer2;
+    double sun_vote;     // carried |r| below this or above outer_vote
+    double outer_vote;   // triggers the exact Sun / outer tests
     uint32_t nb;         // bodies (Sun + planets)
     uint32_t first;      // first particle slot
     uint32_t n;          // particles

@@ -69,7 +69,7 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
         p.rx = p.ry = p.rz = p.vx = p.vy = p.vz = p.ax = p.ay = p.az = 0.0;
         p.flags = 0xFFFFu;  // padding lanes never step
     }
-    p.r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
+    if constexpr (FORM != GLX_FORM_FP64) radius_from_position(p);
 
     for (uint32_t cs = 0; cs < B.steps; cs += S.chunk) {
         const uint32_t cn = min(S.chunk, B.steps - cs);
@@ -78,7 +78,7 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        chunk_steps<0, FORM, kBatchMaxBodies>(p, dti, S, S.pc.m, S.pc.rh2, lds, row, cn, nb, base + cs);
+        chunk_steps<0, FORM, kBatchMaxBodies>(p, dti, S, S.pc.m, S.pc.rh2, lds, row, cn, nb, base + cs, base + cs);
     }
 
     if (valid) {

@@ -381,6 +381,8 @@ int glx_batch_create(int device, const glx_system* systems, uint32_t n_systems, 
         S.sqrt_mu = std::sqrt(y.m[0]);
         S.inv_sqrt_mu = 1.0 / S.sqrt_mu;
         S.outer2 = y.outer_radius * y.outer_radius;
+        S.sun_vote = std::sqrt(S.pc.rh2[0]) * (1.0 + glx::kVoteMargin);
+        S.outer_vote = y.outer_radius * (1.0 - glx::kVoteMargin);
         S.nb = y.n_bodies;
         S.first = (uint32_t)slot;
         S.n = y.n_particles;

@@ -437,6 +437,9 @@ BlockArgs block_args(glx_ctx* c, const PlanetBuf& b, double dt, double outer_rad
     A.sqrt_mu = std::sqrt(b.mu);
     A.inv_sqrt_mu = 1.0 / A.sqrt_mu;
     A.outer2 = outer_radius * outer_radius;
+    A.sun_vote = std::sqrt(b.pc.rh2[0]) * (1.0 + glx::kVoteMargin);
+    A.outer_vote = outer_radius * (1.0 - glx::kVoteMargin);
+    A.block_step0 = 0;
     A.pc = b.pc;
     return A;
 }
@@ -508,15 +511,20 @@ int glx_run_blocks(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, doub
         // slice (the same steps, so the same results): the launch that ends
         // the chain is that much shorter, and so is the time the last
         // partly filled round of workgroups leaves the other SIMDs idle.
-        const uint32_t parts = std::min(launches_per_block, b.n_steps);
+        // Launches start on multiples of kRadiusPeriod steps (the carried
+        // radius is recomputed there anyway), so at most one per period.
+        const uint32_t period = glx::kRadiusPeriod;
+        const uint32_t segs = (b.n_steps + period - 1) / period;
+        const uint32_t parts = std::min(launches_per_block, segs);
         for (uint32_t j = 0; j < parts; j++) {
-            const uint32_t r0 = (uint32_t)((uint64_t)b.n_steps * j / parts);
-            const uint32_t r1 = (uint32_t)((uint64_t)b.n_steps * (j + 1) / parts);
+            const uint32_t r0 = std::min(b.n_steps, (uint32_t)((uint64_t)segs * j / parts) * period);
+            const uint32_t r1 = std::min(b.n_steps, (uint32_t)((uint64_t)segs * (j + 1) / parts) * period);
             BlockArgs A = whole;
             A.log = b.dev_log + r0 * row;
             A.n_steps = r1 - r0;
             A.chunk = std::min(A.chunk, A.n_steps);
             A.step_offset = offset + r0;
+            A.block_step0 = r0;
             for (uint32_t s = 0; s < n_shards; s++) {
                 A.first = s * per;
                 A.n = std::min<uint32_t>(n, (s + 1) * per);

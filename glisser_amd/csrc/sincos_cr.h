@@ -259,6 +259,20 @@ __host__ __device__ inline bool sincos_accurate(double x, double& s, double& c) 
     return ok_s && ok_c;
 }
 
+// Everything but the quick phase, as a call: {sin x, cos x}.  Kept out of
+// line so the rarely taken accurate phase (a few hundred operations and its
+// coefficients) does not occupy the caller's registers at every call site.
+__host__ __device__ __attribute__((noinline)) inline Pair sincos_slow(double x) {
+    double s, c;
+    if (fabs(x) < 0x1p20) {
+        sincos_accurate(x, s, c);
+    } else {
+        s = sin(x);
+        c = cos(x);
+    }
+    return {s, c};
+}
+
 // sin x and cos x, correctly rounded for |x| < 2^20 (except the undecided
 // accurate-phase cases above, never met in the tests).  Below 2^-27 sin x
 // rounds to x and cos x to 1.
@@ -270,12 +284,9 @@ __host__ __device__ inline void sincos_cr(double x, double* s, double* c) {
         return;
     }
     if (ax <= kQuickMax && sincos_quick(x, *s, *c)) return;
-    if (ax < 0x1p20) {
-        sincos_accurate(x, *s, *c);
-        return;
-    }
-    *s = sin(x);
-    *c = cos(x);
+    const Pair r = sincos_slow(x);
+    *s = r.hi;
+    *c = r.lo;
 }
 
 }  // namespace cr

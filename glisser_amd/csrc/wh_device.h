@@ -25,6 +25,16 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kTolKep = 1e-14;  // src/wh.cu:13
 constexpr int kKepIts = 5;         // src/wh.cu:12
 
+// Steps of a block between two recomputations of the FAST / MIXED forms'
+// carried radius from the position.  A launch also recomputes it at its
+// first step, so launches that start on a multiple of this (glx_run_blocks
+// splits blocks there) give the same bits as one launch per block.
+constexpr uint32_t kRadiusPeriod = 64;
+// Relative widening of the radii the carried |r| is voted against (the Sun
+// cull radius and the outer limit), far beyond the carried value's error
+// (< 1e-12 over kRadiusPeriod steps).
+constexpr double kVoteMargin = 1e-9;
+
 // Flag bits (src/data.h:169-180).
 constexpr uint32_t kFlagEncounter = 0x01;
 constexpr uint32_t kFlagOut = 0x02;
@@ -56,6 +66,7 @@ struct BlockArgs {
     uint32_t first;        // particles [first, n) of this launch
     uint32_t n;
     uint32_t step_offset;  // steps of the launch before this block (death index base)
+    uint32_t block_step0;  // index within its block of the launch's first step
     uint32_t n_steps;
     uint32_t chunk;     // steps staged in LDS at a time
     uint32_t n_bodies;  // used by the generic (NB == 0) instantiation
@@ -66,12 +77,14 @@ struct BlockArgs {
     double inv_mu;       // 1/mu, sqrt(mu), 1/sqrt(mu): FAST / MIXED drift
     double sqrt_mu;
     double inv_sqrt_mu;
+    double sun_vote;     // FAST / MIXED: carried |r| below this or above
+    double outer_vote;   // this triggers the exact Sun / outer tests
     PlanetConsts pc;
 };
 
 struct Particle {
     double rx, ry, rz, vx, vy, vz, ax, ay, az;
-    double r2;  // |r|^2 carried between steps (FAST / MIXED forms only)
+    double rl, ir;  // |r| and 1/|r| carried between steps (FAST / MIXED forms only)
     uint32_t flags;
 };
 
@@ -87,7 +100,10 @@ struct Particle {
 // 0x04 if the last Newton correction exceeds 1e-14; for an unbound orbit
 // (energy >= 0) 0x08 with the identity map, so r, v stay unchanged (the
 // reference GPU code computes a NaN orbit there instead, and its CPU twin
-// throws).
+// throws).  kCrTrig false: sin/cos from the device math library instead
+// (the FAST / MIXED forms' fallback lanes, which need no bit-equality with
+// the CPU path and keep their registers for the common path).
+template <bool kCrTrig = true>
 __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double rz, double vx, double vy,
                                                    double vz, double dt, double mu, double& f, double& g,
                                                    double& fdot, double& gdot) {
@@ -111,10 +127,12 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
     const double dM = dtn - kPi * 2 * (double)(int)(dtn / kPi * 2);
     const double adv_dt = dM / n_;
     double sM, cM;
-    cr::sincos_cr(dM, &sM, &cM);
+    if constexpr (kCrTrig) cr::sincos_cr(dM, &sM, &cM);
+    else sincos(dM, &sM, &cM);
     double dE = dM - esinEo + esinEo * cM + ecosEo * sM;
     double sindE, cosdE;
-    cr::sincos_cr(dE, &sindE, &cosdE);
+    if constexpr (kCrTrig) cr::sincos_cr(dE, &sindE, &cosdE);
+    else sincos(dE, &sindE, &cosdE);
     double delta = 0.0;
 #pragma unroll
     for (int k = 0; k < kKepIts; k++) {
@@ -122,7 +140,8 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
         const double fpk = 1. - ecosEo * cosdE + esinEo * sindE;
         delta = -fk / fpk;
         dE += delta;
-        cr::sincos_cr(dE, &sindE, &cosdE);
+        if constexpr (kCrTrig) cr::sincos_cr(dE, &sindE, &cosdE);
+        else sincos(dE, &sindE, &cosdE);
     }
     const double fp = 1.0 - ecosEo * cosdE + esinEo * sindE;
     f = 1.0 + a * (cosdE - 1.0) / dist;

@@ -3,8 +3,11 @@
 // Same map as the reference (kick-drift-kick, eccentric-anomaly-difference
 // Kepler solve, direct planet sum + indirect term, identical flag logic),
 // restructured for the gfx950 FP64 VALU:
-//   * one reciprocal square root of |r|^2 and of 1/a replace the drift's
-//     divisions and square roots; 1/a = -2E/mu needs no division at all;
+//   * one reciprocal square root of 1/a replaces the drift's divisions and
+//     square roots; 1/a = -2E/mu needs no division at all; |r| and 1/|r|
+//     come out of the previous step's Kepler solution (|r(E)| = a F'(E))
+//     and are recomputed from the position every 64 steps of a block and at
+//     the start of a launch;
 //   * reciprocals / reciprocal square roots start from v_rcp_f64 /
 //     v_rsq_f64 (relative error < 6e-8) and take one FMA Newton step where
 //     the result feeds the state (relative error < 6e-15, a few ulp of the
@@ -132,8 +135,21 @@ __device__ __forceinline__ bool newton_step(double esinEo, double ecosEo, double
 // touch (cos E - 1 only enters f, g-dot and F', all of order one).
 constexpr double kTolShort = 1e-20;
 
+// |r| and 1/|r| of the current position, from the position itself.
+__device__ __forceinline__ void radius_from_position(Particle& p) {
+    const double r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
+    p.ir = rsqrt_nr1(r2);
+    p.rl = r2 * p.ir;
+}
+
 // Kepler drift, FAST form.  inv_mu, sqrt_mu, inv_sqrt_mu: host constants;
-// p.r2 must hold |r|^2 (fma(z, z, fma(y, y, x * x))) of the current position.
+// p.rl, p.ir must hold |r| and 1/|r| of the current position (carried from
+// the previous drift or radius_from_position); the drift leaves those of the
+// new position: |r(E)| = a F'(E) = |r0| + a (esinEo sin dE - ecosEo (cos dE
+// - 1)), accumulated as that increment so an error of a (the refined
+// reciprocal square root's, ~4e-15) only scales the increment, and
+// 1/|r(E)| = (1/a) / F'(E).  A relative error d of the carried values moves
+// the new state by O(d dE^2) only (f and g depend on a and e at that order).
 //
 // Straight-line path for the common case, then wave votes: lanes whose
 // first correction is not provably converged recompute sin / cos of E and
@@ -147,10 +163,9 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
                                                double sqrt_mu, double inv_sqrt_mu) {
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
     const double vx0 = p.vx, vy0 = p.vy, vz0 = p.vz;
-    const double r2 = p.r2;  // |r|^2, left by the previous step's planet sum
     const double v2 = fma(vz0, vz0, fma(vy0, vy0, vx0 * vx0));
     const double vdotr = fma(vz0, rz0, fma(vy0, ry0, vx0 * rx0));
-    const double idist = rsqrt_nr1(r2);
+    const double idist = p.ir;
     // 1/a = -2E/mu = 2/|r| - v^2/mu; bound orbits have 1/a > 0
     const double ia = fma(-v2, inv_mu, 2.0 * idist);
     const double rs_ia = rsqrt_nr1(ia);            // sqrt(a)
@@ -158,7 +173,7 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     const double sq_ia = ia * rs_ia;               // sqrt(1/a)
     const double n_ = sqrt_mu * ia * sq_ia;        // sqrt(mu / a^3)
     const double in_ = inv_sqrt_mu * a * rs_ia;    // 1/n
-    const double omec = r2 * idist * ia;           // 1 - e cos E0 = |r| / a
+    const double omec = p.rl * ia;                 // 1 - e cos E0 = |r| / a
     const double ecosEo = 1.0 - omec;
     const double esinEo = vdotr * (inv_sqrt_mu * sq_ia);  // r.v / (n a^2)
     const double q = a * idist;                    // a / |r| = 1 / (1 - e cos E0)
@@ -185,9 +200,10 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     // exact (Sterbenz) while E1 / dM lies in [1/2, 2], which holds for
     // e < 1/3; otherwise its rounding is one more part of the start's error
     double dd = E1 - dM;
-    // lanes for the reference-arithmetic drift, as a wave mask (one compare
-    // and a scalar OR per condition)
-    uint64_t bad = wave_mask(!(ia > 0.0)) | wave_mask(!(fabs(dM) <= kMaxDM));
+    // lanes for the reference-arithmetic drift, as a wave mask: an unbound
+    // orbit (1/a <= 0) makes sqrt(1/a), hence n and dM, NaN, so one compare
+    // also catches it
+    uint64_t bad = wave_mask(!(fabs(dM) <= kMaxDM));
     // sin / cos - 1 of the start: three Taylor terms suffice for |E1| <= 0.02
     // (the regime of the long-period swarms this path is built for); a lane
     // with a larger start takes the retry below, which evaluates five terms
@@ -224,10 +240,14 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
         bad |= wave_mask(far) | wave_mask(!conv);
     }
     const double dE = dM + dd;
-    fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
+    // F'(E) - (1 - e cos E0) = esinEo sin dE - ecosEo (cos dE - 1)
+    const double dfp = fma(esinEo, s, -ecosEo * cm1);
+    fp = omec + dfp;
     // 1/F' at the solution: one Newton step from the reciprocal of the last
     // correction's F', which differs from it by |d| e relative at most
     const double ifp = fma(r, fma(-fp, r, 1.0), r);
+    p.rl = fma(a, dfp, p.rl);
+    p.ir = ia * ifp;
     double fg = fma(cm1, q, 1.0);
     double g = fma(s - dE, in_, dt);
     double fdot = -(n_ * q) * s * ifp;
@@ -236,7 +256,7 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     // reference-arithmetic drift (and its 0x04 / 0x08 flags) instead; the
     // state itself is then updated once, in place, for every lane.
     if (bad != 0) {
-        if (lane_in(bad)) p.flags = kepler_fg_fp64(rx0, ry0, rz0, vx0, vy0, vz0, dt, mu, fg, g, fdot, gdot);
+        if (lane_in(bad)) p.flags = kepler_fg_fp64<false>(rx0, ry0, rz0, vx0, vy0, vz0, dt, mu, fg, g, fdot, gdot);
     }
     p.vx = fma(rx0, fdot, vx0 * gdot);
     p.vy = fma(ry0, fdot, vy0 * gdot);
@@ -244,6 +264,9 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     p.rx = fma(rx0, fg, vx0 * g);
     p.ry = fma(ry0, fg, vy0 * g);
     p.rz = fma(rz0, fg, vz0 * g);
+    if (bad != 0) {
+        if (lane_in(bad)) radius_from_position(p);
+    }
     return bad;
 }
 
@@ -262,15 +285,17 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
 // the particle is still unflagged, then the Sun and the outer test
 // overwrite), but the main path only ORs "some radius crossed" into one
 // wave vote; the exact assignment runs in the rare waves where a lane
-// crossed one.  |r|^2 of the new position is left in p.r2 for the next
-// drift (the kick in between does not move the particle).  hm, hrh2: the
-// GM and squared radii read on the main path (the values of pc.m, pc.rh2).
+// crossed one.  The Sun and outer tests vote on the carried |r| against
+// radii widened by 1e-9 relative (sun_vote, outer_vote), far beyond its
+// error, and the exact tests of |r|^2 run in the voting waves, so the flags
+// are those of the exact tests.  hm, hrh2: the GM and squared radii read on
+// the main path (the values of pc.m, pc.rh2).
 // Returns the wave mask of lanes inside some radius (the only ones it can
 // flag).
 template <int NB, bool MIXED, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES)>
 __device__ __forceinline__ uint64_t accel_vote(Particle& p, const double* __restrict__ row, const PlanetConsts& pc,
                                                const double* __restrict__ hm, const double* __restrict__ hrh2,
-                                               double outer2, uint32_t nb) {
+                                               double sun_vote, double outer_vote, double outer2, uint32_t nb) {
     double ax = row[0], ay = row[1], az = row[2];
     const uint32_t nbodies = NB > 0 ? (uint32_t)NB : nb;
     uint64_t close = 0;  // wave mask: lanes inside some planet's radius
@@ -301,10 +326,9 @@ __device__ __forceinline__ uint64_t accel_vote(Particle& p, const double* __rest
     p.ax = ax;
     p.ay = ay;
     p.az = az;
-    const double r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
-    p.r2 = r2;
-    const uint64_t event = close | wave_mask(r2 < hrh2[0]) | wave_mask(r2 > outer2);
+    const uint64_t event = close | wave_mask(p.rl < sun_vote) | wave_mask(p.rl > outer_vote);
     if (event != 0) {
+        const double r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
         if (lane_in(close) && p.flags == 0) {
             for (uint32_t j = 1; j < nbodies; j++) {
                 const double dx = p.rx - row[3 * j];
@@ -348,11 +372,13 @@ __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A
         for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
         kick_fma(p, A.half_dt);
         event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, hm, hrh2, A.outer2, nb);
+        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, hm, hrh2, A.sun_vote, A.outer_vote,
+                                                              A.outer2, nb);
     } else {
         kick_fma(p, A.half_dt);
         event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
-        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, hm, hrh2, A.outer2, nb);
+        event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, hm, hrh2, A.sun_vote, A.outer_vote,
+                                                              A.outer2, nb);
     }
     kick_fma(p, A.half_dt);
     if (__builtin_expect(event != 0, 0)) {
@@ -381,30 +407,41 @@ __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const 
 }
 
 // Steps [0, cn) of a staged chunk whose rows start at `rows` (row: doubles
-// per step); death0 + s + 1 is step s's death index.  In the FAST / MIXED
-// forms a wave with no flagged lane steps without the per-lane skip, two
-// steps per iteration: with neither the skip's merge nor a loop-carried
-// copy, the state registers of consecutive steps alternate.  Once one of
-// its lanes is flagged (and for an odd last step) the wave finishes the
-// chunk step by step.
+// per step); death0 + s + 1 is step s's death index, g0 + s its index within
+// its block.  In the FAST / MIXED forms a wave with no flagged lane steps
+// without the per-lane skip, two steps per iteration: with neither the
+// skip's merge nor a loop-carried copy, the state registers of consecutive
+// steps alternate.  Once one of its lanes is flagged (and for an odd last
+// step) the wave finishes the chunk step by step.  The carried radius is
+// recomputed from the position before every step of the block that is a
+// multiple of kRadiusPeriod (a wave-uniform test).
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
 __device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C& A, const double* hm,
                                             const double* hrh2, const double* rows, uint32_t row, uint32_t cn,
-                                            uint32_t nb, uint32_t death0) {
+                                            uint32_t nb, uint32_t death0, uint32_t g0) {
+    constexpr uint32_t kMask = kRadiusPeriod - 1;
+    static_assert((kRadiusPeriod & kMask) == 0, "power of two");
     uint32_t s = 0;
     if constexpr (FORM != GLX_FORM_FP64) {
         if (wave_mask(p.flags != 0) == 0) {
             while (s + 1 < cn) {
+                if (((g0 + s) & kMask) == 0) radius_from_position(p);
                 bool flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
                 s++;
                 if (flagged) break;
+                if (((g0 + s) & kMask) == 0) radius_from_position(p);
                 flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
                 s++;
                 if (flagged) break;
             }
         }
     }
-    for (; s < cn; s++) particle_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
+    for (; s < cn; s++) {
+        if constexpr (FORM != GLX_FORM_FP64) {
+            if (((g0 + s) & kMask) == 0) radius_from_position(p);
+        }
+        particle_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
+    }
 }
 
 }  // namespace glx

@@ -46,7 +46,8 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         p.rx = p.ry = p.rz = p.vx = p.vy = p.vz = p.ax = p.ay = p.az = 0.0;
         p.flags = 0xFFFFu;  // padding lanes never step
     }
-    p.r2 = fma(p.rz, p.rz, fma(p.ry, p.ry, p.rx * p.rx));
+
+    if constexpr (FORM != GLX_FORM_FP64) radius_from_position(p);
 
     // With a compile-time body count of at most kRegBodies, the FAST / MIXED
     // forms hold the bodies' GM and squared radii in vector registers: the
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        chunk_steps<NB, FORM>(p, dti, A, pm, prh2, lds, row, cn, nb, A.step_offset + cs);
+        chunk_steps<NB, FORM>(p, dti, A, pm, prh2, lds, row, cn, nb, A.step_offset + cs, A.block_step0 + cs);
     }
 
     if (valid) {

@@ -154,8 +154,9 @@ int glx_run_block(glx_ctx* ctx, int buf, double dt, double outer_radius, int for
  * SIMD (3..8) through the LDS request, so that a SIMD freed early picks up
  * the next ready launch of another slice; 0 leaves occupancy uncapped.
  * launches_per_block (1..64, 0 means 1) runs each block's steps as that
- * many consecutive launches (same results; a shorter last launch shortens
- * the partly idle end of the chain).  deathtime_index becomes the step
+ * many consecutive launches, split on multiples of 64 steps (so at most
+ * ceil(Time-Block-Size / 64) of them; same results; a shorter last launch
+ * shortens the partly idle end of the chain).  deathtime_index becomes the step
  * count within the launch (k * Time-Block-Size + step + 1).  With n_blocks
  * = 1, n_shards = 1 and one launch per block it is glx_run_block.  No
  * reference counterpart: the reference launches one block per host
