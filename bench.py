@@ -102,13 +102,28 @@ def flops_per_particle_step(n_planets: int) -> int:
     return (135 + 16 * n_planets) + (16 + n_planets) + (2 + n_planets) + 14
 
 
-def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int, n_workgroups: int) -> int:
-    """Algorithmic HBM bytes of one time-block: particle state read and
-    written once (r, v, a: 72 B; flags 2 B; deathtime index 4 B) plus the
-    planet block read once per workgroup (n_bodies x 24 B per step)."""
-    state = n_particles * (72 + 2 + 4) * 2
-    planets = n_workgroups * tbsize * n_bodies * 24
-    return state + planets
+def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int) -> int:
+    """Minimum HBM bytes of one time-block: the particle state read and
+    written once (r, v, a: 72 B; flags 2 B; deathtime index 4 B) and the
+    planet block read once (n_bodies x 24 B per step; the workgroups'
+    further reads of it come from cache)."""
+    return n_particles * (72 + 2 + 4) * 2 + tbsize * n_bodies * 24
+
+
+# Source files of the time-block kernel; their SHA-256 when the counters
+# above were taken.  The counter-based fields are reported only while the
+# kernel is the profiled one.
+KERNEL_SOURCES = ["glisser_amd/csrc/wh_kernels.hip", "glisser_amd/csrc/wh_fast.h", "glisser_amd/csrc/wh_device.h",
+                  "glisser_amd/csrc/sincos_cr.h", "glisser_amd/csrc/launch.h"]
+PROFILED_KERNEL_SHA256 = ""
+
+
+def kernel_fingerprint() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        h.update((ROOT / rel).read_bytes())
+    return h.hexdigest()
 
 
 HIST_SPACING = 365.0
@@ -126,7 +141,11 @@ def make_inputs(n_particles: int, n_blocks: int, seed: int):
     return s, hist, rh
 
 
-def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_simd: int, parts: int = 1):
+def prepare_gpu(inputs, steps: int, warmup: int):
+    """A context holding the swarm, the planet history, the initial
+    accelerations and flags, compacted: everything before the first block.
+    bench.py prepares every form's context before timing any of them, so the
+    forms then run back to back on a GPU at its sustained clock."""
     from glisser_amd import native
 
     s, hist, rh = inputs
@@ -140,6 +159,24 @@ def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_s
         ctx.init_accel(s.m, rh, pr0[0], h00[0], OUTER)
         ctx.kill_encounters()
         ctx.partition()
+        ctx.sync()
+    except BaseException:
+        ctx.close()
+        raise
+    return ctx
+
+
+def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_simd: int, parts: int = 1,
+            ctx=None):
+    """`warmup` untimed blocks, then the timed region: the planet rows of
+    `steps` blocks interpolated on the device, one launch of the `steps`
+    blocks, the encounter-kill transform, the step count and the stable
+    partition.  Closes the context."""
+    s, hist, rh = inputs
+    n = s.n
+    if ctx is None:
+        ctx = prepare_gpu(inputs, steps, warmup)
+    try:
         if warmup:
             for k in range(warmup):
                 ctx.interp_block(k, k * TBSIZE * DT, DT, TBSIZE, rh)
@@ -632,11 +669,12 @@ def main(argv=None):
     n_planets = inputs[0].n_bodies - 1
     finals = {}
     f_alg = flops_per_particle_step(n_planets)
-    workgroups = (args.particles + 255) // 256
     per_form = {}
+    counters_current = PROFILED_KERNEL_SHA256 == kernel_fingerprint()
+    ctxs = {form: prepare_gpu(inputs, args.steps, args.warmup) for form in forms}
     for form in forms:
         r = run_gpu(inputs, form, args.steps, args.warmup, args.shards, waves_per_simd(args, form),
-                    args.launches_per_block)
+                    args.launches_per_block, ctx=ctxs.pop(form))
         finals[form] = r.pop("final")
         block_s = r["kernel_ms"] / 1e3 / args.steps
         per_block = r["particle_steps"] / args.steps
@@ -650,7 +688,7 @@ def main(argv=None):
             "particle_steps": r["particle_steps"],
         }
         cf = COUNTER_FLOPS_PER_PSTEP.get(form)
-        if cf:
+        if cf and counters_current:
             entry["counter_fp64_frac"] = per_block * cf / block_s / 1e12 / FP64_PEAK_TFLOPS
         per_form[form] = entry
     h = per_form[head]
@@ -665,7 +703,9 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32" if head == "mixed" else "f64",
+        "dtype_note": ("mixed form: FP64 state, Kepler drift, sums and tests; each planet's r^-3 seeded by an FP32 "
+                       "reciprocal square root and corrected in FP64" if head == "mixed" else "FP64 throughout"),
         "data": "synthetic (seeded Sun + 4 giants, 200k-particle swarm at 60-65 au)",
         "config": {
             "workload": "wh-timeblock-200k",
@@ -692,11 +732,15 @@ def main(argv=None):
             "unit": "TFLOP/s",
             "frac": h["alg_tflops"] / FP64_PEAK_TFLOPS,
             "traffic": (TRAFFIC_BYTES_PER_BLOCK[head] * args.steps
-                        if head in TRAFFIC_BYTES_PER_BLOCK and args.particles == N_PARTICLES else None),
+                        if head in TRAFFIC_BYTES_PER_BLOCK and args.particles == N_PARTICLES and counters_current
+                        else None),
             "traffic_note": "HBM bytes per launch of `steps` blocks (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
             "counter_fp64_frac": h.get("counter_fp64_frac"),
+            "counters_note": ("from the counters of profiles/wh_block_200k_counters.md (same kernel sources)"
+                              if counters_current else
+                              "kernel sources differ from the profiled build: counter-based fields omitted"),
             "flops_per_particle_step": f_alg,
-            "hbm_gbs_algorithmic": hbm_bytes_per_block(args.particles, inputs[0].n_bodies, TBSIZE, workgroups)
+            "hbm_gbs_algorithmic": hbm_bytes_per_block(args.particles, inputs[0].n_bodies, TBSIZE)
             / (h["kernel_ms_per_block"] / 1e3) / 1e9,
         },
     }
@@ -704,6 +748,8 @@ def main(argv=None):
         line["mixed_vs_fp64"] = per_form["mixed"]["value"] / per_form["fp64"]["value"]
     if "fp64" in per_form and "fast" in per_form:
         line["fast_vs_fp64"] = per_form["fast"]["value"] / per_form["fp64"]["value"]
+    if "fast" in per_form and "mixed" in per_form:
+        line["mixed_vs_fast"] = per_form["mixed"]["value"] / per_form["fast"]["value"]
     if "fp64" in finals:
         line["precision_vs_fp64"] = {
             "after_steps": (args.warmup + args.steps) * TBSIZE,

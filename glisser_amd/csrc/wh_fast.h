@@ -357,12 +357,19 @@ __device__ __forceinline__ void kick_fma(Particle& p, double half_dt) {
 // pc, outer2); hm, hrh2: the bodies' GM and squared radii as the planet sum
 // reads them every step (pc's own arrays, or copies held in registers);
 // rowp: the step's planet row in LDS; death: the death index a particle
-// flagged in this step records.  Only the rare branches that can
-// set a flag (the reference-arithmetic drift, a radius crossed) look at
-// the flags.  Returns whether some lane of the wave was flagged.
+// flagged in this step records.  kick_in: the kick before the drift,
+// dt / 2 at the first step of a run of steps, else dt (the previous step's
+// closing half kick and this step's opening one in one operation: the
+// FAST / MIXED forms leave a step's closing half kick pending, see
+// chunk_steps).  A lane flagged in this step gets its closing half kick at
+// once, as the reference completes the flagging step.  Only the rare
+// branches that can set a flag (the reference-arithmetic drift, a radius
+// crossed) look at the flags.  Returns whether some lane of the wave was
+// flagged.
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
 __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A, const double* hm,
-                                          const double* hrh2, const double* rowp, uint32_t nb, uint32_t death) {
+                                          const double* hrh2, const double* rowp, uint32_t nb, uint32_t death,
+                                          double kick_in) {
     uint64_t event;
     if constexpr (NB > 0) {
         // the step's planet row is read from LDS before the drift, so the
@@ -370,30 +377,33 @@ __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A
         double rowv[3 * NB];
 #pragma unroll
         for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
-        kick_fma(p, A.half_dt);
+        kick_fma(p, kick_in);
         event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
         event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, hm, hrh2, A.sun_vote, A.outer_vote,
                                                               A.outer2, nb);
     } else {
-        kick_fma(p, A.half_dt);
+        kick_fma(p, kick_in);
         event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
         event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, hm, hrh2, A.sun_vote, A.outer_vote,
                                                               A.outer2, nb);
     }
-    kick_fma(p, A.half_dt);
     if (__builtin_expect(event != 0, 0)) {
-        if (p.flags != 0) dti = death;
+        if (p.flags != 0) {
+            dti = death;
+            kick_fma(p, A.half_dt);
+        }
         return wave_mask(p.flags != 0) != 0;
     }
     return false;
 }
 
 // One step of one particle, any form (skipped once it is flagged); the
-// arguments are those of fast_step.
+// arguments are those of fast_step (kick_in is not used by the fp64 form,
+// which kicks twice per step as the reference does).
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
 __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const C& A, const double* hm,
                                               const double* hrh2, const double* rowp, uint32_t nb,
-                                              uint32_t death) {
+                                              uint32_t death, double kick_in) {
     if (p.flags != 0) return;
     if constexpr (FORM == GLX_FORM_FP64) {
         kick(p, A.half_dt);
@@ -402,8 +412,16 @@ __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const 
         kick(p, A.half_dt);
         if (p.flags != 0) dti = death;
     } else {
-        fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rowp, nb, death);
+        fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rowp, nb, death, kick_in);
     }
+}
+
+// The closing half kick left pending by the FAST / MIXED steps, for every
+// lane not flagged (flagged lanes completed their last step when flagged,
+// and lanes flagged before the launch did not step).
+__device__ __forceinline__ void close_kick(Particle& p, double half_dt, bool& pending) {
+    if (pending && p.flags == 0) kick_fma(p, half_dt);
+    pending = false;
 }
 
 // Steps [0, cn) of a staged chunk whose rows start at `rows` (row: doubles
@@ -412,25 +430,43 @@ __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const 
 // without the per-lane skip, two steps per iteration: with neither the
 // skip's merge nor a loop-carried copy, the state registers of consecutive
 // steps alternate.  Once one of its lanes is flagged (and for an odd last
-// step) the wave finishes the chunk step by step.  The carried radius is
-// recomputed from the position before every step of the block that is a
-// multiple of kRadiusPeriod (a wave-uniform test).
+// step) the wave finishes the chunk step by step.
+//
+// FAST / MIXED: every step of the block that is a multiple of
+// kRadiusPeriod starts a run of steps: the pending closing half kick of
+// the step before is applied and the carried radius recomputed from the
+// position (wave-uniform tests).  Within a run, a step's closing half kick
+// and the next step's opening one are one kick by dt.  `pending` (wave
+// uniform) says whether the last step's closing half kick is still due; the
+// caller applies it (close_kick) at the end of the launch, so a launch ends,
+// like every run, on completed steps, and launches split on multiples of
+// kRadiusPeriod give the same bits as one launch.
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
 __device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C& A, const double* hm,
                                             const double* hrh2, const double* rows, uint32_t row, uint32_t cn,
-                                            uint32_t nb, uint32_t death0, uint32_t g0) {
+                                            uint32_t nb, uint32_t death0, uint32_t g0, bool& pending) {
     constexpr uint32_t kMask = kRadiusPeriod - 1;
     static_assert((kRadiusPeriod & kMask) == 0, "power of two");
     uint32_t s = 0;
     if constexpr (FORM != GLX_FORM_FP64) {
         if (wave_mask(p.flags != 0) == 0) {
             while (s + 1 < cn) {
-                if (((g0 + s) & kMask) == 0) radius_from_position(p);
-                bool flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
+                if (((g0 + s) & kMask) == 0) {
+                    close_kick(p, A.half_dt, pending);
+                    radius_from_position(p);
+                }
+                bool flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1,
+                                                         pending ? A.dt : A.half_dt);
+                pending = true;
                 s++;
                 if (flagged) break;
-                if (((g0 + s) & kMask) == 0) radius_from_position(p);
-                flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
+                if (((g0 + s) & kMask) == 0) {
+                    close_kick(p, A.half_dt, pending);
+                    radius_from_position(p);
+                }
+                flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1,
+                                                    pending ? A.dt : A.half_dt);
+                pending = true;
                 s++;
                 if (flagged) break;
             }
@@ -438,9 +474,14 @@ __device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C&
     }
     for (; s < cn; s++) {
         if constexpr (FORM != GLX_FORM_FP64) {
-            if (((g0 + s) & kMask) == 0) radius_from_position(p);
+            if (((g0 + s) & kMask) == 0) {
+                close_kick(p, A.half_dt, pending);
+                radius_from_position(p);
+            }
         }
-        particle_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1);
+        particle_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1,
+                                      pending ? A.dt : A.half_dt);
+        if constexpr (FORM != GLX_FORM_FP64) pending = true;
     }
 }
 

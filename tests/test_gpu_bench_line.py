@@ -30,7 +30,7 @@ def test_bench_json_line():
     d = json.loads(lines[0])
     assert all(k in d for k in KEYS), [k for k in KEYS if k not in d]
     assert d["n_gpus"] == 1 and d["steps"] == 2 and d["warmup"] == 1 and d["higher_is_better"] is True
-    assert d["value"] > 0 and d["ms_per_step"] > 0 and d["dtype"] == "f64"
+    assert d["value"] > 0 and d["ms_per_step"] > 0 and d["dtype"] == "f64+f32"
     assert set(d["forms"]) == {"fp64", "fast", "mixed"}
     assert d["value"] == d["forms"]["mixed"]["value"]
     rf = d["roofline"]
