@@ -71,6 +71,11 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
     }
     if constexpr (FORM != GLX_FORM_FP64) radius_from_position(p);
 
+    TaylorLead lead = taylor_lead();
+    if constexpr (FORM != GLX_FORM_FP64) {
+        pin(lead.s7);
+        pin(lead.c6);
+    }
     bool pending = false;  // FAST / MIXED: closing half kick still due
     for (uint32_t cs = 0; cs < B.steps; cs += S.chunk) {
         const uint32_t cn = min(S.chunk, B.steps - cs);
@@ -80,7 +85,7 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
         chunk_steps<0, FORM, kBatchMaxBodies>(p, dti, S, S.pc.m, S.pc.rh2, lds, row, cn, nb, base + cs, base + cs,
-                                              pending);
+                                              pending, lead);
     }
     if constexpr (FORM != GLX_FORM_FP64) close_kick(p, S.half_dt, pending);
 

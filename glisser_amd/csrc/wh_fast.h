@@ -94,6 +94,25 @@ __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1,
     cm1 = pc * x2;
 }
 
+// Highest coefficients of the three-term start polynomials (x^7 of sin,
+// x^6 of cos - 1), passed by value so a kernel can hold them in vector
+// registers: a fused multiply-add reads one scalar operand, so with both
+// Horner operands in scalar registers one of them is copied to a vector
+// register at every step.
+struct TaylorLead {
+    double s7, c6;
+};
+__device__ __forceinline__ TaylorLead taylor_lead() { return {kTaylor[0][2], kTaylor[1][2]}; }
+
+// sincos_taylor<3> with the highest coefficients from `lead`.
+__device__ __forceinline__ void sincos_taylor3(double x, const TaylorLead& lead, double& s, double& cm1) {
+    const double x2 = x * x;
+    const double ps = fma(fma(lead.s7, x2, kTaylor[0][1]), x2, kTaylor[0][0]);
+    const double pc = fma(fma(lead.c6, x2, kTaylor[1][1]), x2, kTaylor[1][0]);
+    s = fma(ps * x2, x, x);
+    cm1 = pc * x2;
+}
+
 // Validity bound of the FAST drift's shortcut: |dM| (no full turns to
 // remove) and |E| wherever the five-term Taylor sin/cos is evaluated
 // (truncation below 9e-18 absolute at 0.2).
@@ -127,13 +146,15 @@ __device__ __forceinline__ bool newton_step(double esinEo, double ecosEo, double
     return small && d2 < (2.0 * kTolKep) * fp;
 }
 
-// Acceptance bound of the first (short) Kepler correction: d^2 < 1e-20 F',
-// i.e. |d| < 1.5e-10 (F' < 2).  Within it the error the correction leaves
+// Acceptance bound of the first (short) Kepler correction: |d| < 7e-11 and
+// F' > 0.49 (every orbit with e < 0.51 has F' = 1 - e cos E > 0.49), so
+// d^2 < 4.9e-21 < 1e-20 F'.  Within it the error the correction leaves
 // (d^2 / (2 F') < 5e-21), the raw reciprocal's share (|d| x 6e-8 < 1e-17)
 // and the terms its rotation drops (s d^2 / 2 and d^3 / 6 from sin, d^2 / 2
 // from cos - 1: below 1.2e-20) are all under one ulp of the quantities they
 // touch (cos E - 1 only enters f, g-dot and F', all of order one).
-constexpr double kTolShort = 1e-20;
+constexpr double kShortD = 7e-11;
+constexpr double kShortFp = 0.49;
 
 // |r| and 1/|r| of the current position, from the position itself.
 __device__ __forceinline__ void radius_from_position(Particle& p) {
@@ -160,7 +181,7 @@ __device__ __forceinline__ void radius_from_position(Particle& p) {
 // Returns the wave mask of those lanes (the only ones whose flags the drift
 // can set).
 __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu, double inv_mu,
-                                               double sqrt_mu, double inv_sqrt_mu) {
+                                               double sqrt_mu, double inv_sqrt_mu, const TaylorLead& lead) {
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
     const double vx0 = p.vx, vy0 = p.vy, vz0 = p.vz;
     const double v2 = fma(vz0, vz0, fma(vy0, vy0, vx0 * vx0));
@@ -208,10 +229,10 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     // (the regime of the long-period swarms this path is built for); a lane
     // with a larger start takes the retry below, which evaluates five terms
     double s, cm1;
-    sincos_taylor<3>(E1, s, cm1);
+    sincos_taylor3(E1, lead, s, cm1);
     // First correction, short form: raw reciprocal and the rotation
     // sin(E + d) = s + d (1 + cm1), cos(E + d) - 1 = cm1 - s d, accepted only
-    // within kTolShort.  A Newton correction d leaves an error of at most
+    // within kShortD, kShortFp.  A Newton correction d leaves an error of at most
     // |F''| d^2 / (2 F') <= d^2 / (2 F') (|F''| <= e < 1).
     double f = fma(-esinEo, cm1, fma(-ecosEo, s, dd));
     double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
@@ -221,7 +242,8 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     const double s1 = fma(cm1, d, s + d);
     cm1 = fma(-s, d, cm1);
     s = s1;
-    const uint64_t retry = wave_mask(!(d * d < kTolShort * fp)) | wave_mask(!(fabs(E1) <= kMaxStart));
+    const uint64_t retry =
+        wave_mask(!(fabs(d) < kShortD)) | wave_mask(!(fp > kShortFp)) | wave_mask(!(fabs(E1) <= kMaxStart));
     // Lanes outside that bound (large steps or eccentric orbits) restart from
     // sin / cos of the corrected E and take full corrections (accurate
     // reciprocal, third-order rotation, the reference's 1e-14 tolerance as a
@@ -369,7 +391,7 @@ __device__ __forceinline__ void kick_fma(Particle& p, double half_dt) {
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
 __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A, const double* hm,
                                           const double* hrh2, const double* rowp, uint32_t nb, uint32_t death,
-                                          double kick_in) {
+                                          double kick_in, const TaylorLead& lead) {
     uint64_t event;
     if constexpr (NB > 0) {
         // the step's planet row is read from LDS before the drift, so the
@@ -378,12 +400,12 @@ __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A
 #pragma unroll
         for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
         kick_fma(p, kick_in);
-        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu, lead);
         event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, hm, hrh2, A.sun_vote, A.outer_vote,
                                                               A.outer2, nb);
     } else {
         kick_fma(p, kick_in);
-        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu);
+        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu, lead);
         event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, hm, hrh2, A.sun_vote, A.outer_vote,
                                                               A.outer2, nb);
     }
@@ -403,7 +425,7 @@ __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
 __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const C& A, const double* hm,
                                               const double* hrh2, const double* rowp, uint32_t nb,
-                                              uint32_t death, double kick_in) {
+                                              uint32_t death, double kick_in, const TaylorLead& lead) {
     if (p.flags != 0) return;
     if constexpr (FORM == GLX_FORM_FP64) {
         kick(p, A.half_dt);
@@ -412,7 +434,7 @@ __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const 
         kick(p, A.half_dt);
         if (p.flags != 0) dti = death;
     } else {
-        fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rowp, nb, death, kick_in);
+        fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rowp, nb, death, kick_in, lead);
     }
 }
 
@@ -444,7 +466,8 @@ __device__ __forceinline__ void close_kick(Particle& p, double half_dt, bool& pe
 template <int NB, int FORM, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), class C>
 __device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C& A, const double* hm,
                                             const double* hrh2, const double* rows, uint32_t row, uint32_t cn,
-                                            uint32_t nb, uint32_t death0, uint32_t g0, bool& pending) {
+                                            uint32_t nb, uint32_t death0, uint32_t g0, bool& pending,
+                                            const TaylorLead& lead) {
     constexpr uint32_t kMask = kRadiusPeriod - 1;
     static_assert((kRadiusPeriod & kMask) == 0, "power of two");
     uint32_t s = 0;
@@ -456,7 +479,7 @@ __device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C&
                     radius_from_position(p);
                 }
                 bool flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1,
-                                                         pending ? A.dt : A.half_dt);
+                                                         pending ? A.dt : A.half_dt, lead);
                 pending = true;
                 s++;
                 if (flagged) break;
@@ -465,7 +488,7 @@ __device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C&
                     radius_from_position(p);
                 }
                 flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1,
-                                                    pending ? A.dt : A.half_dt);
+                                                    pending ? A.dt : A.half_dt, lead);
                 pending = true;
                 s++;
                 if (flagged) break;
@@ -480,7 +503,7 @@ __device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C&
             }
         }
         particle_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1,
-                                      pending ? A.dt : A.half_dt);
+                                      pending ? A.dt : A.half_dt, lead);
         if constexpr (FORM != GLX_FORM_FP64) pending = true;
     }
 }

@@ -71,6 +71,13 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         prh2 = hrh2;
     }
 
+    // FAST / MIXED: the start polynomials' highest coefficients in vector
+    // registers
+    TaylorLead lead = taylor_lead();
+    if constexpr (FORM != GLX_FORM_FP64) {
+        pin(lead.s7);
+        pin(lead.c6);
+    }
     bool pending = false;  // FAST / MIXED: closing half kick still due
     for (uint32_t cs = 0; cs < A.n_steps; cs += A.chunk) {
         const uint32_t cn = min(A.chunk, A.n_steps - cs);
@@ -79,7 +86,8 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        chunk_steps<NB, FORM>(p, dti, A, pm, prh2, lds, row, cn, nb, A.step_offset + cs, A.block_step0 + cs, pending);
+        chunk_steps<NB, FORM>(p, dti, A, pm, prh2, lds, row, cn, nb, A.step_offset + cs, A.block_step0 + cs, pending,
+                              lead);
     }
     if constexpr (FORM != GLX_FORM_FP64) close_kick(p, A.half_dt, pending);
 
