@@ -54,7 +54,8 @@ def oracle_run(oracle, system, h0_0, rh0, blocks, dt, outer=300.0, kill=True, co
     r, v, a, flags, ids = r[perm], v[perm], a[perm], flags[perm], ids[perm]
     n_alive = split
     dti = np.zeros(n, np.uint32)  # indexed by id, as in the reference
-    death = np.zeros(n, np.int64)  # by id: steps done when flagged (0: never)
+    death = np.full(n, -1, np.int64)  # by id: steps done when flagged (-1: never)
+    death[ids[flags != 0]] = 0
     t = 0.0
     done = 0
     for rl, h0, rh in blocks:
@@ -90,7 +91,9 @@ def gpu_run(ctx, system, h0_0, rh0, blocks, dt, outer=300.0, form="fp64", kill=T
         ctx.kill_encounters()
     ctx.partition()
     dti_by_id = np.zeros(n, np.uint32)
-    death = np.zeros(n, np.int64)  # by id: steps done when flagged (0: never)
+    death = np.full(n, -1, np.int64)  # by id: steps done when flagged (-1: never)
+    d0 = ctx.download(n)
+    death[d0["id"][d0["flags"] != 0]] = 0
     done = 0
     for k, (rl, h0, rh) in enumerate(blocks):
         n_alive, _ = ctx.counts()
@@ -115,7 +118,8 @@ def gpu_run(ctx, system, h0_0, rh0, blocks, dt, outer=300.0, form="fp64", kill=T
     return d
 
 
-def oracle_run_parallel(oracle, system, h0_0, rh0, blocks, dt, outer=300.0, correctly_rounded=False, chunks=8):
+def oracle_run_parallel(oracle, system, h0_0, rh0, blocks, dt, outer=300.0, correctly_rounded=False, chunks=8,
+                        kill=True):
     """oracle_run over disjoint particle chunks on `chunks` host threads
     (particles are independent given the planet blocks; the oracle's calls
     release the interpreter lock).  Returns r, v, flags, death_by_id and the
@@ -129,12 +133,12 @@ def oracle_run_parallel(oracle, system, h0_0, rh0, blocks, dt, outer=300.0, corr
         sub = copy.copy(system)
         sub.pr, sub.pv = system.pr[idx], system.pv[idx]
         sub.ids = np.arange(len(idx), dtype=np.uint32)
-        return oracle_run(oracle, sub, h0_0, rh0, blocks, dt, outer, correctly_rounded=correctly_rounded)
+        return oracle_run(oracle, sub, h0_0, rh0, blocks, dt, outer, kill=kill, correctly_rounded=correctly_rounded)
 
     with ThreadPoolExecutor(len(parts)) as pool:
         outs = list(pool.map(one, parts))
     res = {"id": system.ids[order].copy(), "r": np.empty((system.n, 3)), "v": np.empty((system.n, 3)),
-           "flags": np.empty(system.n, np.uint16), "death_by_id": np.zeros(int(system.ids.max()) + 1, np.int64),
+           "flags": np.empty(system.n, np.uint16), "death_by_id": np.full(int(system.ids.max()) + 1, -1, np.int64),
            "libm_calls": 0, "libm_differ": 0}
     pos = 0
     for idx, out in zip(parts, outs):
@@ -175,9 +179,12 @@ def threshold_distance(oracle, system, h0_0, rh0, blocks, dt, outer, pid, steps)
     r, v = system.pr[k:k + 1].copy(), system.pv[k:k + 1].copy()
     ids = system.ids[k:k + 1].copy()
     a, flags = oracle.particle_accels(system.m, rh0, outer, system.r[1:], h0_0, r, np.zeros(1, np.uint16), ids)
+    rows, radii = system.r[1:], rh0  # step 0: the initial positions
     left, t = int(steps), 0.0
     dti = np.zeros(int(system.ids.max()) + 1, np.uint32)
     for rl, h0, rh in blocks:
+        if left == 0:
+            break
         m = min(left, len(h0))
         out = oracle.particle_block(system.m, rh, outer, rl[:m], h0[:m], t, dt, r, v, a, flags, ids, dti)
         r, v, a, flags, dti = out["r"], out["v"], out["a"], out["flags"], out["dti"]
@@ -203,7 +210,7 @@ def unexplained_flag_mismatches(oracle, system, h0_0, rh0, blocks, dt, outer, cp
     bad = np.union1d(low, high)
     dist, unexplained = [], []
     for pid in bad:
-        steps = [x for x in (dc[pid], dg[pid]) if x > 0]
+        steps = [x for x in (dc[pid], dg[pid]) if x >= 0]
         g = threshold_distance(oracle, system, h0_0, rh0, blocks, dt, outer, pid, min(steps))
         dist.append(g)
         if not g < tol:
