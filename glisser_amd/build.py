@@ -2,6 +2,8 @@
 
 * ``glisser_amd/libglisser_amd.so`` -- the HIP kernels and the C ABI, for
   gfx950, with ``hipcc`` (cross-compiles without a GPU).
+* ``tests/cxx/executor_drive`` -- a C++ caller of the C ABI and of
+  ``glx::Executor`` (g++, linked to the library) that the GPU tests run.
 * ``oracle/liboracle.so`` and, where the reference tree exists,
   ``oracle/_ref/libglisser_ref.so`` -- the CPU checker (``make -C oracle``).
 
@@ -76,6 +78,18 @@ def build_hip(out: Path = LIB) -> Path:
     return out
 
 
+CXX_TEST = ROOT / "tests" / "cxx" / "executor_drive"
+
+
+def build_cxx_test(out: Path = CXX_TEST) -> Path:
+    """The C++ caller of the test suite: g++ against include/*.h, linked to
+    the library (found next to it at run time through a relative rpath)."""
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(out.with_suffix(".cpp")),
+          f"-L{PKG}", "-lglisser_amd", "-Wl,-rpath,$ORIGIN/../../glisser_amd", "-o", str(out)])
+    return out
+
+
 def build_oracle() -> None:
     targets = ["oracle"]
     if Path("/root/reference/src/wh.cpp").exists():
@@ -85,6 +99,7 @@ def build_oracle() -> None:
 
 def build_all() -> None:
     build_hip()
+    build_cxx_test()
     build_oracle()
 
 
@@ -97,3 +112,5 @@ if __name__ == "__main__":
             build_hip()
         if "oracle" in what:
             build_oracle()
+        if "cxx" in what:
+            build_cxx_test()

@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <fstream>
+#include <iomanip>
 #include <limits>
 #include <new>
 #include <stdexcept>
@@ -42,7 +44,11 @@ struct Executor::Impl {
     std::unique_ptr<host::PlanetHistory> history;
     glx_ctx* ctx = nullptr;
     uint32_t nb_max = 0;
-    uint64_t n_total = 0, n_alive = 0;
+    // n_alive: the active range the kernel runs on (shrunk at resyncs);
+    // n_split: unflagged particles after the last compaction
+    uint64_t n_total = 0, n_alive = 0, n_split = 0;
+    uint64_t resync_counter = 0;
+    std::ofstream log;
     double t = 0.0;
     int buf = 0;
     // one planet block: start time, step, length, bodies and their radii,
@@ -87,7 +93,16 @@ struct Executor::Impl {
     double planet_ms = 0.0;
     std::string history_end;  // set once a planet history has no block left
 
-    explicit Impl(const ExecutorConfig& c) : cfg(c) {}
+    explicit Impl(const ExecutorConfig& c) : cfg(c) {
+        if (cfg.resync_every < 1) throw std::runtime_error("Executor: resync_every must be at least 1");
+        if (cfg.resync_every != 1 && cfg.resolve_encounters)
+            throw std::runtime_error("Error: Resync-Interval must be 1 when Resolve-Encounters is enabled");
+        if (!cfg.encounter_log.empty()) {
+            log.open(cfg.encounter_log);
+            if (!log) throw std::runtime_error("Executor: cannot open encounter log " + cfg.encounter_log);
+            log << std::setprecision(17);
+        }
+    }
 
     ~Impl() {
         if (ctx) glx_destroy(ctx);
@@ -166,11 +181,17 @@ struct Executor::Impl {
     const glx_encounter* staged = nullptr;
     uint64_t staged_n = 0, staged_enc = 0;
     double staged_t = 0.0, staged_dt = 0.0;
+    bool staged_resync = false;  // the records are those a resync removed
+    double staged_t_end = 0.0;   // end time of that block
 
-    // Stages the records of particles [begin, end) flagged during the block
-    // that started at t_block; in resolve mode the first n_enc of them (the
-    // encounter class) go to the queue with their rollback state.
-    void stage_flagged(uint64_t begin, uint64_t end, uint64_t n_enc, double t_block, double dt_block) {
+    // Stages the records of particles [begin, end) after the compaction of
+    // the block that started at t_block: the particles it flagged or, at a
+    // resync, every particle that leaves the active range (those flagged in
+    // it and in the blocks since the last resync).  In resolve mode the first
+    // n_enc of them (the encounter class) go to the queue with their
+    // rollback state.
+    void stage_flagged(uint64_t begin, uint64_t end, uint64_t n_enc, double t_block, double dt_block, bool resync,
+                       double t_end) {
         const uint64_t n = end - begin;
         if (n == 0) return;
         check(pack_encounters_staged(ctx, begin, n, t_block, &staged), "glx_pack_encounters");
@@ -178,16 +199,32 @@ struct Executor::Impl {
         staged_enc = n_enc;
         staged_t = t_block;
         staged_dt = dt_block;
+        staged_resync = resync;
+        staged_t_end = t_end;
     }
 
     // Moves staged records to the death-time table and the queue, read where
-    // the device copy put them.
+    // the device copy put them.  Death time: the flagging step's time, set
+    // once (a particle flagged in an earlier block of the same resync already
+    // has it), or with reference_deathtime the resync block's end time cast
+    // to float; after a resync also the log lines (src/executor.cu:816-838).
     void take_staged() {
-        for (uint64_t k = 0; k < staged_n; k++)
-            set_deathtime(staged[k].id, staged_t + (double)staged[k].step * staged_dt);
+        const float t_end_f = static_cast<float>(staged_t_end);
+        for (uint64_t k = 0; k < staged_n; k++) {
+            const glx_encounter& e = staged[k];
+            if (!cfg.reference_deathtime) {
+                if (std::isnan(get_deathtime(e.id))) set_deathtime(e.id, staged_t + (double)e.step * staged_dt);
+            } else if (staged_resync) {
+                set_deathtime(e.id, static_cast<double>(t_end_f));
+            }
+            if (staged_resync && log.is_open())
+                log << e.id << ((e.flags & 0x80) ? " death " : " encounter ") << staged_t_end << '\n';
+        }
+        if (staged_resync && log.is_open()) log.flush();
         queue.insert(queue.end(), staged, staged + staged_enc);
         staged = nullptr;
         staged_n = staged_enc = 0;
+        staged_resync = false;
     }
 
     // One time-block: its kernel; then, on the host while the kernel runs,
@@ -212,23 +249,32 @@ struct Executor::Impl {
         take_row(next);
         produce_block(t_block + (double)n_steps * dt_block);
         if (prev) {
+            // After the stable compaction the particles flagged in this block
+            // sit at [new split, split before), followed by those flagged
+            // in the blocks since the last resync; a resync removes all of
+            // them from the active range.
+            const uint64_t before = n_split;
             const uint64_t ne = compact(prev);
-            stage_flagged(n_alive, prev, cfg.resolve_encounters ? ne : 0, t_block, dt_block);
+            const bool resync = ++resync_counter % cfg.resync_every == 0;
+            const double t_end = t_block + (double)n_steps * dt_block;
+            stage_flagged(n_split, resync ? prev : before, cfg.resolve_encounters ? ne : 0, t_block, dt_block, resync,
+                          t_end);
+            if (resync) n_alive = n_split;
         }
         buf ^= 1;
         upload_block();
         t = next.n_steps ? next.t0 : t_block + (double)n_steps * dt_block;
     }
 
-    // Stable compaction of [0, n) (kill first unless resolving); returns
-    // the encounter-class size.
+    // Stable compaction of [0, n) (kill first unless resolving): n_split
+    // becomes the unflagged count; returns the encounter-class size.
     uint64_t compact(uint64_t n) {
         check(glx_set_counts(ctx, n, n_total), "glx_set_counts");
         if (!cfg.resolve_encounters) check(glx_kill_encounters(ctx), "glx_kill_encounters");
         uint64_t na = 0, ne = 0;
         check(glx_partition(ctx, cfg.resolve_encounters ? GLX_PART_THREE_WAY : GLX_PART_UNFLAGGED, &na, &ne),
               "glx_partition");
-        n_alive = na;
+        n_split = na;
         return ne;
     }
 };
@@ -320,9 +366,20 @@ Executor::Executor(const ExecutorConfig& cfg, uint32_t n_bodies, const double* m
                              s.row_h0.data(), cfg.outer_radius),
               "glx_init_accel");
         if (cfg.resolve_encounters) check(glx_snapshot_rollback(s.ctx), "glx_snapshot_rollback");
-        const uint64_t ne = s.compact(n);
-        s.stage_flagged(s.n_alive, n, cfg.resolve_encounters ? ne : 0, cfg.t0, cfg.dt);
-        s.take_staged();
+        if (cfg.reference_deathtime && !cfg.resolve_encounters) {
+            // as Executor::init (src/executor.cu:180-195): particles flagged
+            // for an encounter at t0 stay in the active range until the first
+            // block's kill and resync; the others never get a death time
+            uint64_t na = 0, ne = 0;
+            check(glx_partition(s.ctx, GLX_PART_THREE_WAY, &na, &ne), "glx_partition");
+            s.n_split = na;
+            s.n_alive = na + ne;
+        } else {
+            const uint64_t ne = s.compact(n);
+            s.n_alive = s.n_split;
+            s.stage_flagged(s.n_alive, n, cfg.resolve_encounters ? ne : 0, cfg.t0, cfg.dt, false, cfg.t0);
+            s.take_staged();
+        }
     }
     s.produce_block(cfg.t0);
     s.upload_block();
@@ -404,7 +461,7 @@ void Executor::return_particles(uint64_t n, const uint32_t* ids, const double* r
           "glx_upload_particles");
     check(glx_set_counts(s.ctx, s.n_total, s.n_total), "glx_set_counts");
     check(glx_partition(s.ctx, GLX_PART_THREE_WAY, &na, &ne), "glx_partition");
-    s.n_alive = na;
+    s.n_alive = s.n_split = na;
     check(glx_set_counts(s.ctx, s.n_alive, s.n_total), "glx_set_counts");
 }
 
@@ -482,6 +539,9 @@ int glx_exec_create(const glx_exec_params* p, uint32_t n_bodies, const double* m
         c.device = p->device;
         if (p->planet_history) c.planet_history = p->planet_history;
         c.history_table_size = p->history_table_size ? p->history_table_size : 16;
+        c.resync_every = p->resync_every ? p->resync_every : 1;
+        c.reference_deathtime = p->reference_deathtime != 0;
+        if (p->encounter_log) c.encounter_log = p->encounter_log;
         auto e = std::make_unique<glx_exec>();
         e->ex = std::make_unique<glx::Executor>(c, n_bodies, m, r, v, planet_id, n, pr, pv, ids);
         *out = e.release();

@@ -114,6 +114,9 @@ class ExecParams(C.Structure):
         ("device", C.c_int32),
         ("planet_history", C.c_char_p),
         ("history_table_size", C.c_uint32),
+        ("resync_every", C.c_uint32),
+        ("reference_deathtime", C.c_int32),
+        ("encounter_log", C.c_char_p),
     ]
 
 
@@ -604,9 +607,13 @@ class Executor:
     def __init__(self, m, r, v, pr, pv, ids=None, planet_ids=None, t0: float = 0.0, dt: float = 180.0,
                  tbsize: int = 2048, outer_radius: float = 300.0, cull_radius: float = 0.5, rh_factor: float = 0.0,
                  resolve_encounters: bool = False, form: str | int = "fp64", device: int = 0,
-                 planet_history=None, history_table_size: int = 16):
+                 planet_history=None, history_table_size: int = 16, resync_every: int = 1,
+                 reference_deathtime: bool = False, encounter_log=None):
         """m, r, v: planets (Sun first); with planet_history (a history file)
-        they are not used and may be None."""
+        they are not used and may be None.  resync_every: Resync-Interval;
+        reference_deathtime: death times as the reference records them (end
+        of the resync block, cast to float); encounter_log: a file for the
+        reference's encounter.out lines."""
         self.lib = load()
         if planet_history is None:
             m = np.ascontiguousarray(m, dtype=np.float64)
@@ -623,8 +630,9 @@ class Executor:
         pid = None if planet_ids is None else np.ascontiguousarray(planet_ids, dtype=np.uint32)
         f = FORMS[form] if isinstance(form, str) else int(form)
         hist = None if planet_history is None else str(planet_history).encode()
+        log = None if encounter_log is None else str(encounter_log).encode()
         p = ExecParams(t0, dt, tbsize, outer_radius, cull_radius, rh_factor, int(resolve_encounters), f, device, hist,
-                       history_table_size)
+                       history_table_size, resync_every, int(reference_deathtime), log)
         h = C.c_void_p()
         rc = self.lib.glx_exec_create(C.byref(p), nb, _ptr(m, _dp), _ptr(r, _dp), _ptr(v, _dp),
                                       _ptr(pid, _u32p), n, _ptr(pr, _dp), _ptr(pv, _dp), _ptr(ids, _u32p),

@@ -367,6 +367,10 @@ typedef struct glx_exec_params {
     int32_t device;
     const char* planet_history; /* Read-Planet-History file, or NULL: integrate the planets given */
     uint32_t history_table_size; /* Planet-History-Max-Planet-Count (0 means 16) */
+    uint32_t resync_every;      /* Resync-Interval (0 means 1); 1 with resolve_encounters */
+    int32_t reference_deathtime; /* death time = end of the resync block cast to float, as the reference;
+                                    0: the time of the flagging step */
+    const char* encounter_log;  /* NULL, or a file for the reference's encounter.out lines */
 } glx_exec_params;
 
 /* Create the loop: planets (n_bodies GM values, heliocentric r, v; unused
@@ -389,7 +393,9 @@ int glx_exec_take_encounters(glx_exec* e, uint64_t capacity, glx_encounter* out,
 /* Hand resolved particles back at the loop's current time (r, v: n x 3). */
 int glx_exec_return(glx_exec* e, uint64_t n, const uint32_t* ids, const double* r, const double* v);
 /* Every particle slot (n_total): r, v (n x 3), flags, id, deathtime (time of
- * the flagging step, NaN if never flagged); any output may be NULL. */
+ * the flagging step, or with reference_deathtime the end time of the block
+ * whose resync removed the particle, cast to float; NaN if not set); any
+ * output may be NULL. */
 int glx_exec_download(glx_exec* e, double* r, double* v, uint16_t* flags, uint32_t* id, double* deathtime);
 /* Heliocentric planet positions ((n_bodies - 1) x 3, n_bodies from
  * glx_exec_bodies, at most GLX_MAX_BODIES) at the current time. */

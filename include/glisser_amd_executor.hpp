@@ -42,6 +42,22 @@ struct ExecutorConfig {
     // of being integrated; empty = integrate the planets given.
     std::string planet_history;
     uint32_t history_table_size = 16;  // Planet-History-Max-Planet-Count
+    // Resync-Interval: the active range shrinks, and the particles that left
+    // it get their death times and log lines, every resync_every blocks (the
+    // compaction itself runs every block, as the reference's presort does);
+    // must be 1 with resolve_encounters, as the reference requires
+    // (src/data.cpp:317-320, src/executor.cu:729-735).
+    uint32_t resync_every = 1;
+    // Death time as the reference's resync records it: the end time of the
+    // block at whose resync the particle left the active range, cast to float
+    // (src/executor.cu:816-822).  false: the time of the flagging step.
+    bool reference_deathtime = false;
+    // When not empty, the file of "id death t" / "id encounter t" lines the
+    // reference writes to encounter.out at every resync for the particles
+    // that left the active range ("death" when the 0x80 bit is set), t the
+    // end time of that block with 17 significant digits
+    // (src/executor.cu:205-208,828-838; targets/main.cpp:125-126).
+    std::string encounter_log;
 };
 
 // One particle handed to the host close-encounter queue: id, flags (0x01,
@@ -52,7 +68,7 @@ using EncounterRecord = glx_encounter;
 
 // Host copy of every particle slot (r, v: n x 3).
 struct ParticleState {
-    std::vector<double> r, v, deathtime;  // deathtime: time of the flagging step (NaN if never flagged)
+    std::vector<double> r, v, deathtime;  // deathtime: see ExecutorConfig::reference_deathtime (NaN if not set)
     std::vector<uint16_t> flags;
     std::vector<uint32_t> id;
     uint64_t n_alive = 0;
@@ -71,7 +87,7 @@ public:
     Executor(const Executor&) = delete;
     Executor& operator=(const Executor&) = delete;
 
-    // One time-block (Executor::loop + resync2).
+    // One time-block (Executor::loop, and resync2 every resync_every blocks).
     void step();
     void run(uint32_t n_blocks);
 
