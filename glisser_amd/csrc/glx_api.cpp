@@ -120,6 +120,9 @@ struct glx_ctx {
     glx_encounter* enc_dev = nullptr;  // encounter-record staging (grown on demand)
     glx_encounter* enc_pin = nullptr;
     uint64_t enc_cap = 0;
+    hipStream_t dth_s = nullptr;      // device -> host copies of staged records
+    hipEvent_t enc_packed = nullptr;  // records packed on the main stream
+    hipEvent_t enc_copied = nullptr;  // records in the pinned staging buffer
     std::vector<void*> allocs;
 };
 
@@ -156,6 +159,7 @@ void release(glx_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->main_s) (void)hipStreamSynchronize(c->main_s);
     if (c->htd_s) (void)hipStreamSynchronize(c->htd_s);
+    if (c->dth_s) (void)hipStreamSynchronize(c->dth_s);
     for (uint32_t k = 0; k < kMaxShards; k++)
         if (c->shard_s[k]) (void)hipStreamSynchronize(c->shard_s[k]);
     for (void* p : c->allocs) (void)hipFree(p);
@@ -177,6 +181,9 @@ void release(glx_ctx* c) {
         if (c->shard_done[k]) (void)hipEventDestroy(c->shard_done[k]);
         if (c->shard_s[k]) (void)hipStreamDestroy(c->shard_s[k]);
     }
+    for (hipEvent_t e : {c->enc_packed, c->enc_copied})
+        if (e) (void)hipEventDestroy(e);
+    if (c->dth_s) (void)hipStreamDestroy(c->dth_s);
     if (c->main_s) (void)hipStreamDestroy(c->main_s);
     if (c->htd_s) (void)hipStreamDestroy(c->htd_s);
     delete c;
@@ -281,8 +288,12 @@ int glx_create(const glx_params* p, glx_ctx** out) {
         return code;
     };
     if (hipSetDevice(c->device) != hipSuccess) return bail(fail(GLX_EHIP, "hipSetDevice failed"));
-    if (hipStreamCreate(&c->main_s) != hipSuccess || hipStreamCreate(&c->htd_s) != hipSuccess)
+    if (hipStreamCreate(&c->main_s) != hipSuccess || hipStreamCreate(&c->htd_s) != hipSuccess ||
+        hipStreamCreate(&c->dth_s) != hipSuccess)
         return bail(fail(GLX_EHIP, "hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&c->enc_packed, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->enc_copied, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(GLX_EHIP, "hipEventCreate failed"));
     for (uint32_t k = 0; k < kMaxShards; k++)
         if (hipStreamCreate(&c->shard_s[k]) != hipSuccess ||
             hipEventCreateWithFlags(&c->shard_done[k], hipEventDisableTiming) != hipSuccess)
@@ -920,32 +931,56 @@ int glx_track_pack(glx_ctx* c, uint32_t n_planets, const double* pm, const doubl
 
 namespace glx {
 
-int pack_encounters_staged(glx_ctx* c, uint64_t begin, uint64_t n, double t_block, const glx_encounter** out) {
-    if (!c) return fail(GLX_EINVAL, "null context");
-    if (!out) return fail(GLX_EINVAL, "null output");
-    *out = nullptr;
-    if (int rc = check_range(c, begin, n)) return rc;
-    if (n == 0) return GLX_OK;
-    GLX_HIP(hipSetDevice(c->device));
-    if (n > c->enc_cap) {
-        GLX_HIP(hipStreamSynchronize(c->main_s));
-        if (c->enc_dev) (void)hipFree(c->enc_dev);
-        if (c->enc_pin) (void)hipHostFree(c->enc_pin);
-        c->enc_dev = nullptr;
-        c->enc_pin = nullptr;
-        c->enc_cap = 0;
-        const uint64_t cap = std::max<uint64_t>(n, 4096);
-        if (hipMalloc((void**)&c->enc_dev, cap * sizeof(glx_encounter)) != hipSuccess ||
-            hipHostMalloc((void**)&c->enc_pin, cap * sizeof(glx_encounter), hipHostMallocDefault) != hipSuccess)
-            return fail(GLX_ENOMEM, "encounter staging of %llu records failed", (unsigned long long)cap);
-        c->enc_cap = cap;
-    }
-    GLX_HIP(glx::launch_pack_encounters(c->cur, c->keep_rb ? &c->rb : nullptr, (uint32_t)begin, (uint32_t)n, t_block,
-                                        c->enc_dev, c->main_s));
-    GLX_HIP(hipMemcpyAsync(c->enc_pin, c->enc_dev, n * sizeof(glx_encounter), hipMemcpyDeviceToHost, c->main_s));
+// Grows the record staging to hold n records (waits for the context's
+// queued work first: the buffers may be in use).
+int grow_encounter_staging(glx_ctx* c, uint64_t n) {
+    if (n <= c->enc_cap) return GLX_OK;
     GLX_HIP(hipStreamSynchronize(c->main_s));
+    GLX_HIP(hipStreamSynchronize(c->dth_s));
+    if (c->enc_dev) (void)hipFree(c->enc_dev);
+    if (c->enc_pin) (void)hipHostFree(c->enc_pin);
+    c->enc_dev = nullptr;
+    c->enc_pin = nullptr;
+    c->enc_cap = 0;
+    const uint64_t cap = std::max<uint64_t>(n, 4096);
+    if (hipMalloc((void**)&c->enc_dev, cap * sizeof(glx_encounter)) != hipSuccess ||
+        hipHostMalloc((void**)&c->enc_pin, cap * sizeof(glx_encounter), hipHostMallocDefault) != hipSuccess)
+        return fail(GLX_ENOMEM, "encounter staging of %llu records failed", (unsigned long long)cap);
+    c->enc_cap = cap;
+    return GLX_OK;
+}
+
+int pack_encounters_begin(glx_ctx* c, uint64_t begin, uint64_t n, double t_block) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_range(c, begin, n)) return rc;
+    GLX_HIP(hipSetDevice(c->device));
+    if (int rc = grow_encounter_staging(c, n)) return rc;
+    if (n > 0) {
+        GLX_HIP(glx::launch_pack_encounters(c->cur, c->keep_rb ? &c->rb : nullptr, (uint32_t)begin, (uint32_t)n,
+                                            t_block, c->enc_dev, c->main_s));
+        GLX_HIP(hipEventRecord(c->enc_packed, c->main_s));
+        GLX_HIP(hipStreamWaitEvent(c->dth_s, c->enc_packed, 0));
+        GLX_HIP(hipMemcpyAsync(c->enc_pin, c->enc_dev, n * sizeof(glx_encounter), hipMemcpyDeviceToHost,
+                               c->dth_s));
+    }
+    GLX_HIP(hipEventRecord(c->enc_copied, c->dth_s));
+    return GLX_OK;
+}
+
+int pack_encounters_wait(glx_ctx* c, const glx_encounter** out) {
+    if (!c || !out) return fail(GLX_EINVAL, "null argument");
+    GLX_HIP(hipSetDevice(c->device));
+    GLX_HIP(hipEventSynchronize(c->enc_copied));
     *out = c->enc_pin;
     return GLX_OK;
+}
+
+int pack_encounters_staged(glx_ctx* c, uint64_t begin, uint64_t n, double t_block, const glx_encounter** out) {
+    if (!out) return fail(GLX_EINVAL, "null output");
+    *out = nullptr;
+    if (int rc = pack_encounters_begin(c, begin, n, t_block)) return rc;
+    if (n == 0) return GLX_OK;
+    return pack_encounters_wait(c, out);
 }
 
 }  // namespace glx

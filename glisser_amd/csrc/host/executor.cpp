@@ -67,7 +67,10 @@ struct Executor::Impl {
     uint32_t row_nb = 0;
     std::vector<double> row_r, row_h0, row_m, row_rh;
     std::vector<uint32_t> row_id;
-    std::vector<EncounterRecord> queue;
+    // the host close-encounter queue, one chunk per staging (no regrowth
+    // of one large buffer while records arrive)
+    std::vector<std::vector<EncounterRecord>> queue;
+    uint64_t queued = 0;
     // time of the flagging step by particle id: a flat table when the ids
     // are dense (max id < 4 n), else a hash map
     std::vector<double> deathtime_flat;
@@ -175,10 +178,10 @@ struct Executor::Impl {
         row_h0.assign(b.log_h0.begin() + (size_t)(b.n_steps - 1) * 3, b.log_h0.begin() + (size_t)b.n_steps * 3);
     }
 
-    // Records of the particles a block flagged, packed on the device into
-    // the context's pinned staging buffer and not yet moved to the queue and
-    // the death-time table (the next block's kernel runs meanwhile).
-    const glx_encounter* staged = nullptr;
+    // Records of the particles a block flagged, packed on the device and on
+    // their way to the context's pinned staging buffer, not yet moved to the
+    // queue and the death-time table (the next block's kernel runs
+    // meanwhile).
     uint64_t staged_n = 0, staged_enc = 0;
     double staged_t = 0.0, staged_dt = 0.0;
     bool staged_resync = false;  // the records are those a resync removed
@@ -194,7 +197,7 @@ struct Executor::Impl {
                        double t_end) {
         const uint64_t n = end - begin;
         if (n == 0) return;
-        check(pack_encounters_staged(ctx, begin, n, t_block, &staged), "glx_pack_encounters");
+        check(pack_encounters_begin(ctx, begin, n, t_block), "glx_pack_encounters");
         staged_n = n;
         staged_enc = n_enc;
         staged_t = t_block;
@@ -209,6 +212,9 @@ struct Executor::Impl {
     // has it), or with reference_deathtime the resync block's end time cast
     // to float; after a resync also the log lines (src/executor.cu:816-838).
     void take_staged() {
+        if (staged_n == 0) return;
+        const glx_encounter* staged = nullptr;
+        check(pack_encounters_wait(ctx, &staged), "glx_pack_encounters");
         const float t_end_f = static_cast<float>(staged_t_end);
         for (uint64_t k = 0; k < staged_n; k++) {
             const glx_encounter& e = staged[k];
@@ -221,8 +227,10 @@ struct Executor::Impl {
                 log << e.id << ((e.flags & 0x80) ? " death " : " encounter ") << staged_t_end << '\n';
         }
         if (staged_resync && log.is_open()) log.flush();
-        queue.insert(queue.end(), staged, staged + staged_enc);
-        staged = nullptr;
+        if (staged_enc) {
+            queue.emplace_back(staged, staged + staged_enc);
+            queued += staged_enc;
+        }
         staged_n = staged_enc = 0;
         staged_resync = false;
     }
@@ -406,12 +414,36 @@ void Executor::run(uint32_t n_blocks) {
 double Executor::time() const { return p_->t; }
 uint64_t Executor::n_alive() const { return p_->n_alive; }
 uint64_t Executor::n_total() const { return p_->n_total; }
-uint64_t Executor::n_queued() const { return p_->queue.size(); }
+uint64_t Executor::n_queued() const { return p_->queued; }
 
 std::vector<EncounterRecord> Executor::take_encounters() {
+    Impl& s = *p_;
     std::vector<EncounterRecord> out;
-    out.swap(p_->queue);
+    if (s.queue.size() == 1) {
+        out.swap(s.queue[0]);
+    } else {
+        out.reserve(s.queued);
+        for (const auto& q : s.queue) out.insert(out.end(), q.begin(), q.end());
+    }
+    s.queue.clear();
+    s.queued = 0;
     return out;
+}
+
+uint64_t Executor::take_encounters(EncounterRecord* out, uint64_t capacity) {
+    Impl& s = *p_;
+    const uint64_t n = s.queued;
+    if (n > capacity || (n && !out))
+        throw std::invalid_argument("encounter queue holds " + std::to_string(n) + " records, capacity " +
+                                    std::to_string(capacity));
+    EncounterRecord* o = out;
+    for (const auto& q : s.queue) {
+        std::memcpy(o, q.data(), q.size() * sizeof(EncounterRecord));
+        o += q.size();
+    }
+    s.queue.clear();
+    s.queued = 0;
+    return n;
 }
 
 void Executor::return_particles(uint64_t n, const uint32_t* ids, const double* r, const double* v) {
@@ -581,9 +613,7 @@ int glx_exec_take_encounters(glx_exec* e, uint64_t capacity, glx_encounter* out,
             return glx::set_error(GLX_EINVAL, "encounter queue holds " + std::to_string(n) + " records, capacity " +
                                                   std::to_string(capacity));
         }
-        auto recs = e->ex->take_encounters();
-        if (n) std::memcpy(out, recs.data(), n * sizeof(glx_encounter));
-        *n_out = n;
+        *n_out = e->ex->take_encounters(out, capacity);
         return GLX_OK;
     });
 }

@@ -98,6 +98,9 @@ public:
 
     // Take every queued record (the queue is emptied).
     std::vector<EncounterRecord> take_encounters();
+    // The same into out[0, n_queued()), which must hold them all (else
+    // nothing is taken and an exception says so); returns the count.
+    uint64_t take_encounters(EncounterRecord* out, uint64_t capacity);
     // Hand resolved particles back at the current time(): heliocentric r, v
     // (n x 3) of particles previously taken from the queue.  Their
     // accelerations are recomputed from the planets at time() and an
