@@ -70,6 +70,14 @@ TRAFFIC_BYTES_PER_BLOCK = {"mixed": 133.1e6}
 TRACK_RECORD_BYTES = 28  # u32 id + 6 float32 per particle in a track snapshot
 
 
+_T_START = time.perf_counter()
+
+
+def note(msg: str) -> None:
+    """One progress line on stderr (the JSON line goes to stdout)."""
+    print(f"bench [{time.perf_counter() - _T_START:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def copy_probe(n: int = 1_000_000, repeats: int = 3) -> None:
     """Calibration kernel for the HBM byte counters: the rollback snapshot
     copies r, v, flags, deathtime index and id (58 B per particle) of n
@@ -342,9 +350,11 @@ def run_encounter(forms, n_particles: int, n_blocks: int, seed: int) -> dict:
             n0, t_s = st["n_alive"], st["t"]
             # one call for all timed blocks: each block's records are taken
             # on the host while the next block's kernel runs
+            k0, _ = ex.kernel_total()
             t0 = time.perf_counter()
             ex.step(n_blocks)  # the records are in the host queue when this returns
             el = time.perf_counter() - t0
+            k1, nk = ex.kernel_total()
             rec = ex.take_encounters()
             handed = len(rec["id"])
             st = ex.status()
@@ -366,6 +376,7 @@ def run_encounter(forms, n_particles: int, n_blocks: int, seed: int) -> dict:
         drift = np.abs(c1 - c0)[ok] / np.abs(c0[ok])
         out["forms"][form] = {
             "value": psteps / el, "unit": "particle-steps/s", "ms_per_block": el / n_blocks * 1e3,
+            "kernel_ms_per_block": (k1 - k0) / n_blocks, "host_overhead": el * 1e3 / (k1 - k0) - 1.0,
             "kernel_ms_last_block": kernel_ms, "partition_ms_last_block": part_ms, "other_flagged": other_flagged,
             "planet_block_host_ms": planet_ms, "alive_start": int(n0), "alive_end": int(d["n_alive"]),
             "handed_to_host": handed, "handed_warmup_block": queued0,
@@ -664,6 +675,7 @@ def main(argv=None):
     rank = int(os.environ.get("RANK", "0"))
 
     inputs = make_inputs(args.particles, args.warmup + args.steps, args.seed)
+    note(f"inputs ready: {args.particles} particles, {args.warmup} + {args.steps} blocks")
     forms = ["fp64", "fast", "mixed"] if args.form == "all" else [args.form]
     head = HEADLINE_FORM if HEADLINE_FORM in forms else forms[0]
     n_planets = inputs[0].n_bodies - 1
@@ -687,6 +699,8 @@ def main(argv=None):
             "elapsed_s": r["elapsed_s"],
             "particle_steps": r["particle_steps"],
         }
+        note(f"{form}: {entry['value']:.4g} particle-steps/s, {entry['kernel_ms_per_block']:.3f} ms per block "
+             "in the kernel")
         cf = COUNTER_FLOPS_PER_PSTEP.get(form)
         if cf and counters_current:
             entry["counter_fp64_frac"] = per_block * cf / block_s / 1e12 / FP64_PEAK_TFLOPS
@@ -756,23 +770,29 @@ def main(argv=None):
             **{f: element_differences(inputs[0].m[0], finals["fp64"], finals[f]) for f in finals if f != "fp64"}}
     if not args.no_batch:
         line["batch"] = run_batch(forms, args.batch_systems, args.batch_launches, args.seed + 2)
+        note("batch done")
     if not args.no_encounter:
         line["encounter"] = run_encounter(forms, args.encounter_particles, ENC_BLOCKS, args.seed + 3)
+        note("encounter hand-off done")
     c0_final = None
     if not args.no_config0:
         line["config0"] = run_config0(forms)
+        note("config-0 case done")
         c0_final = line["config0"].pop("final", None)
     records = None
     if not args.no_track:
         line["track"] = run_track(inputs)
+        note("track done")
         records = line["track"].pop("records")
     if rank == 0 and not args.no_cpu:
         # the CPU leg: the reference's CPU path timed on the host and, as the
         # checker, the oracle over a particle sample of the timed run
         cpu = cpu_baseline(inputs, args.cpu_sample, threads=args.cpu_threads)
+        note("CPU baseline done")
         if args.oracle_sample > 0:
             n_s = min(args.oracle_sample, args.particles)
             ref = oracle_sample_run(inputs, n_s, args.warmup + args.steps)
+            note("oracle sample done")
             cpu["precision_vs_oracle"] = {
                 "after_steps": (args.warmup + args.steps) * TBSIZE, "sample": n_s,
                 **{f: element_differences(inputs[0].m[0], ref, subset_by_id(finals[f], ref["id"])) for f in finals}}

@@ -183,6 +183,9 @@ struct Executor::Impl {
     // queue and the death-time table (the next block's kernel runs
     // meanwhile).
     uint64_t staged_n = 0, staged_enc = 0;
+    // device time of every block kernel so far, and their count
+    double kernel_ms_total = 0.0;
+    uint64_t kernel_blocks = 0;
     double staged_t = 0.0, staged_dt = 0.0;
     bool staged_resync = false;  // the records are those a resync removed
     double staged_t_end = 0.0;   // end time of that block
@@ -263,6 +266,10 @@ struct Executor::Impl {
             // them from the active range.
             const uint64_t before = n_split;
             const uint64_t ne = compact(prev);
+            float k_ms = 0.0f;  // the kernel ended before the compaction's counts came back
+            check(glx_last_times(ctx, &k_ms, nullptr), "glx_last_times");
+            kernel_ms_total += k_ms;
+            kernel_blocks++;
             const bool resync = ++resync_counter % cfg.resync_every == 0;
             const double t_end = t_block + (double)n_steps * dt_block;
             stage_flagged(n_split, resync ? prev : before, cfg.resolve_encounters ? ne : 0, t_block, dt_block, resync,
@@ -527,6 +534,11 @@ void Executor::last_times(double* kernel_ms, double* partition_ms, double* plane
     if (planet_ms) *planet_ms = p_->planet_ms;
 }
 
+double Executor::kernel_ms_total(uint64_t* n_blocks) const {
+    if (n_blocks) *n_blocks = p_->kernel_blocks;
+    return p_->kernel_ms_total;
+}
+
 }  // namespace glx
 
 // ---------------------------------------------------------------- C ABI ----
@@ -659,6 +671,12 @@ int glx_exec_bodies(glx_exec* e, uint32_t* n_bodies, double* m, uint32_t* id) {
         const auto v = e->ex->planet_ids();
         std::memcpy(id, v.data(), v.size() * sizeof(uint32_t));
     }
+    return GLX_OK;
+}
+
+int glx_exec_kernel_total(glx_exec* e, double* kernel_ms, uint64_t* n_blocks) {
+    if (!e || !kernel_ms) return glx::set_error(GLX_EINVAL, "null argument");
+    *kernel_ms = e->ex->kernel_ms_total(n_blocks);
     return GLX_OK;
 }
 

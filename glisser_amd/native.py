@@ -187,6 +187,7 @@ SIGNATURES = {
     "glx_exec_planets": (C.c_int, [C.c_void_p, _dp]),
     "glx_exec_bodies": (C.c_int, [C.c_void_p, _u32p, _dp, _u32p]),
     "glx_exec_times": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "glx_exec_kernel_total": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_uint64)]),
     "glx_config_normalize": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint64, _u64p]),
     "glx_state_save": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(StateC)]),
     "glx_state_sizes": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_uint64, _u32p, _u64p]),
@@ -721,6 +722,12 @@ class Executor:
         k, q, p = C.c_double(), C.c_double(), C.c_double()
         self._check(self.lib.glx_exec_times(self.h, C.byref(k), C.byref(q), C.byref(p)))
         return k.value, q.value, p.value
+
+    def kernel_total(self) -> tuple[float, int]:
+        """Device time of every block kernel so far (ms) and their count."""
+        k, n = C.c_double(), C.c_uint64()
+        self._check(self.lib.glx_exec_kernel_total(self.h, C.byref(k), C.byref(n)))
+        return k.value, n.value
 
 
 def _state_c(st: dict):

@@ -406,6 +406,11 @@ int glx_exec_planets(glx_exec* e, double* r);
 int glx_exec_bodies(glx_exec* e, uint32_t* n_bodies, double* m, uint32_t* id);
 /* Device time of the last kernel and compaction, host time of the last planet block (ms). */
 int glx_exec_times(glx_exec* e, double* kernel_ms, double* partition_ms, double* planet_ms);
+/* Device time of every block kernel since glx_exec_create (ms) and the
+ * number of blocks that ran one (n_blocks may be NULL).  The reference
+ * executor keeps no such total; it is for callers that time the host loop
+ * against its kernels. */
+int glx_exec_kernel_total(glx_exec* e, double* kernel_ms, uint64_t* n_blocks);
 
 /* ---------------------------------------------------------------------
  * File formats of the reference (configuration, state, binary track):

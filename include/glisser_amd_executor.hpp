@@ -120,6 +120,9 @@ public:
     // Device time of the last kernel and compaction, host time of the last
     // planet block (ms).
     void last_times(double* kernel_ms, double* partition_ms, double* planet_ms) const;
+    // Device time of every block kernel since construction (ms); *n_blocks
+    // (if given) the number of blocks that ran one.
+    double kernel_ms_total(uint64_t* n_blocks = nullptr) const;
 
 private:
     struct Impl;
