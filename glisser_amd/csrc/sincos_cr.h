@@ -115,8 +115,10 @@ constexpr double kP2 = 0x1.1a62633145c07p-54;
 constexpr double kP3 = -0x1.f1976b7ed8fbcp-110;
 constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
 
-// Quick-phase bound and its error bounds (see sincos_quick).
+// Quick-phase bound and its error bounds (see sincos_quick); below
+// kQuickShort the quick phase uses two fewer polynomial terms per function.
 constexpr double kQuickMax = 0x1p-4;
+constexpr double kQuickShort = 0x1p-5;
 constexpr double kQuickSinErr = 0x1p-70;  // relative to |x|
 constexpr double kQuickCosErr = 0x1p-69;  // absolute (cos x ~ 1)
 // Accurate-phase error bounds: relative to the result (series and pair
@@ -146,7 +148,12 @@ __host__ __device__ inline bool round_decided(double hi, double lo, double err, 
 // kQuickSinErr = 2^-70 |x| is used.  cos: u = -x^2/2 (exact pair) + x^4
 // C(x^2), C = c4 + z (c6 + z (c8 + z (c10 + z c12))) in double: |err| <
 // 6 * 2^-53 * x^4 / 24 < 2^-71 at |x| = 2^-4, dropped x^14 / 14! < 2^-100;
-// kQuickCosErr = 2^-69.  Returns false when either result is undecided.
+// kQuickCosErr = 2^-69.  For |x| <= 2^-5 the polynomials stop at s9 and c8:
+// the dropped sin terms are below x^11 / 11! < 2^-75 |x| and the dropped cos
+// terms below x^10 / 10! < 2^-71.8, the rounding errors shrink with x, so
+// the same two bounds hold.  (A decided result is the correctly rounded
+// value whichever polynomial produced it, so the choice never changes the
+// returned bits.)  Returns false when either result is undecided.
 __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
 #pragma clang fp contract(off)
     const double x2 = x * x;
@@ -156,20 +163,26 @@ __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
     const double x3l = fma(x, x2, -x3) + x * x2l;
     const double t1 = kS3h * x3;
     const double t1l = fma(kS3h, x3, -t1) + fma(kS3h, x3l, kS3l * x3);
-    double ps = fma(x2, kS13h, kS11h);
-    ps = fma(ps, x2, kS9h);
-    ps = fma(ps, x2, kS7h);
-    ps = fma(ps, x2, kS5h);
+    double ps, pc;
+    if (fabs(x) <= kQuickShort) {
+        ps = fma(fma(x2, kS9h, kS7h), x2, kS5h);
+        pc = fma(fma(x2, kC8h, kC6h), x2, kC4h);
+    } else {
+        ps = fma(x2, kS13h, kS11h);
+        ps = fma(ps, x2, kS9h);
+        ps = fma(ps, x2, kS7h);
+        ps = fma(ps, x2, kS5h);
+        pc = fma(x2, kC12h, kC10h);
+        pc = fma(pc, x2, kC8h);
+        pc = fma(pc, x2, kC6h);
+        pc = fma(pc, x2, kC4h);
+    }
     const double t2 = (x3 * x2) * ps;
     const Pair th = fast_two_sum(t1, t2);
     const Pair sh = fast_two_sum(x, th.hi);
     const double slo = sh.lo + (th.lo + t1l);
     // cos
     const double h = -0.5 * x2, hl = -0.5 * x2l;
-    double pc = fma(x2, kC12h, kC10h);
-    pc = fma(pc, x2, kC8h);
-    pc = fma(pc, x2, kC6h);
-    pc = fma(pc, x2, kC4h);
     const double u2 = (x2 * x2) * pc;
     const Pair uh = fast_two_sum(h, u2);
     const Pair ch = fast_two_sum(1.0, uh.hi);

@@ -122,9 +122,13 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
     const double n_ = sqrt(mu / (a * a * a));
     const double ecosEo = 1.0 - dist / a;
     const double esinEo = vdotr / (n_ * a * a);
-    // dt*n - (pi*2)*(int)((dt*n/pi)*2): the reference's M_2PI macro expansion
+    // dt*n - (pi*2)*(int)((dt*n/pi)*2): the reference's M_2PI macro
+    // expansion.  For 0 <= dt*n < 1.5 the quotient lies below 0.955, the
+    // integer part is 0 and the expression is dt*n itself (dt*n - 0.0), so
+    // the divide only runs outside that range.
     const double dtn = dt * n_;
-    const double dM = dtn - kPi * 2 * (double)(int)(dtn / kPi * 2);
+    double dM = dtn;
+    if (!(dtn >= 0.0 && dtn < 1.5)) dM = dtn - kPi * 2 * (double)(int)(dtn / kPi * 2);
     const double adv_dt = dM / n_;
     double sM, cM;
     if constexpr (kCrTrig) cr::sincos_cr(dM, &sM, &cM);
