@@ -12,7 +12,7 @@
 // mathematical functions with correctly rounded last bit", ACM TOMS 17(3),
 // 1991) with error-free transformations (Knuth's TwoSum, Dekker's
 // Fast2Sum, the FMA two-product):
-//   * quick phase, |x| <= 2^-4, no argument reduction: sin x = x + t and
+//   * quick phase, |x| <= 2^-2, no argument reduction: sin x = x + t and
 //     cos x = 1 + u with the leading Taylor term of t (x^3 / 6) and of u
 //     (x^2 / 2) exact in double-double and the rest in double; the sum is
 //     held as an unevaluated pair (hi, lo) whose error bound D is proven
@@ -115,12 +115,16 @@ constexpr double kP2 = 0x1.1a62633145c07p-54;
 constexpr double kP3 = -0x1.f1976b7ed8fbcp-110;
 constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
 
-// Quick-phase bound and its error bounds (see sincos_quick); below
-// kQuickShort the quick phase uses two fewer polynomial terms per function.
-constexpr double kQuickMax = 0x1p-4;
+// Quick-phase bounds and error bounds (see sincos_quick): three tiers of
+// polynomial length, |x| <= kQuickShort, <= kQuickTight (both with the tight
+// error bounds) and <= kQuickMax (one more term, looser bounds).
+constexpr double kQuickMax = 0x1p-2;
+constexpr double kQuickTight = 0x1p-4;
 constexpr double kQuickShort = 0x1p-5;
 constexpr double kQuickSinErr = 0x1p-70;  // relative to |x|
 constexpr double kQuickCosErr = 0x1p-69;  // absolute (cos x ~ 1)
+constexpr double kWideSinErr = 0x1p-64;   // the same for 2^-4 < |x| <= 2^-2
+constexpr double kWideCosErr = 0x1p-62;
 // Accurate-phase error bounds: relative to the result (series and pair
 // arithmetic) plus the absolute error of the argument reduction.
 constexpr double kAccErr = 0x1p-98;
@@ -137,7 +141,7 @@ __host__ __device__ inline bool round_decided(double hi, double lo, double err, 
     return up == dn;
 }
 
-// Quick phase for |x| <= 2^-4 (x not tiny).  Error analysis, sin:
+// Quick phase for |x| <= 2^-2 (x not tiny).  Error analysis for |x| <= 2^-4, sin:
 // t = x^3 s3 + x^5 P(x^2), the first term as a pair from the exact x^2
 // (x2 + x2l), the exact-enough x^3 (x3 + x3l) and s3 as a pair (errors
 // ~2^-104 |x^3|); P(z) = s5 + z (s7 + z (s9 + z (s11 + z s13))) in double,
@@ -151,9 +155,14 @@ __host__ __device__ inline bool round_decided(double hi, double lo, double err, 
 // kQuickCosErr = 2^-69.  For |x| <= 2^-5 the polynomials stop at s9 and c8:
 // the dropped sin terms are below x^11 / 11! < 2^-75 |x| and the dropped cos
 // terms below x^10 / 10! < 2^-71.8, the rounding errors shrink with x, so
-// the same two bounds hold.  (A decided result is the correctly rounded
-// value whichever polynomial produced it, so the choice never changes the
-// returned bits.)  Returns false when either result is undecided.
+// the same two bounds hold.  For 2^-4 < |x| <= 2^-2 the polynomials run to
+// s15 and c14: the same sums give |err| < 9 * 2^-53 * |x|^5 / 119 <
+// 2^-64.7 |x| for sin (dropped x^17 / 17! < 2^-80 |x|) and 7 * 2^-53 * x^4 /
+// 24 < 2^-62.8 for cos (dropped x^16 / 16! < 2^-76), so kWideSinErr =
+// 2^-64 |x| and kWideCosErr = 2^-62, which leave about one call in 200
+// undecided (2 err / ulp: 2^-10 for sin, 2^-8 for cos).  (A decided result is the correctly rounded value whichever
+// polynomial produced it, so the choice never changes the returned bits.)
+// Returns false when either result is undecided.
 __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
 #pragma clang fp contract(off)
     const double x2 = x * x;
@@ -163,10 +172,24 @@ __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
     const double x3l = fma(x, x2, -x3) + x * x2l;
     const double t1 = kS3h * x3;
     const double t1l = fma(kS3h, x3, -t1) + fma(kS3h, x3l, kS3l * x3);
-    double ps, pc;
-    if (fabs(x) <= kQuickShort) {
+    const double ax = fabs(x);
+    double ps, pc, es = kQuickSinErr, ec = kQuickCosErr;
+    if (ax <= kQuickShort) {
         ps = fma(fma(x2, kS9h, kS7h), x2, kS5h);
         pc = fma(fma(x2, kC8h, kC6h), x2, kC4h);
+    } else if (ax > kQuickTight) {
+        ps = fma(x2, kS15h, kS13h);
+        ps = fma(ps, x2, kS11h);
+        ps = fma(ps, x2, kS9h);
+        ps = fma(ps, x2, kS7h);
+        ps = fma(ps, x2, kS5h);
+        pc = fma(x2, kC14h, kC12h);
+        pc = fma(pc, x2, kC10h);
+        pc = fma(pc, x2, kC8h);
+        pc = fma(pc, x2, kC6h);
+        pc = fma(pc, x2, kC4h);
+        es = kWideSinErr;
+        ec = kWideCosErr;
     } else {
         ps = fma(x2, kS13h, kS11h);
         ps = fma(ps, x2, kS9h);
@@ -187,8 +210,8 @@ __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
     const Pair uh = fast_two_sum(h, u2);
     const Pair ch = fast_two_sum(1.0, uh.hi);
     const double clo = ch.lo + (uh.lo + hl);
-    const bool ok_s = round_decided(sh.hi, slo, kQuickSinErr * fabs(x), s);
-    const bool ok_c = round_decided(ch.hi, clo, kQuickCosErr, c);
+    const bool ok_s = round_decided(sh.hi, slo, es * ax, s);
+    const bool ok_c = round_decided(ch.hi, clo, ec, c);
     return ok_s && ok_c;
 }
 

@@ -30,13 +30,15 @@ def _args(seed=0):
     parts = [
         rng.uniform(0.004, 0.009, 4000),                    # Kepler steps at 60-65 au, dt = 180 d
         np.exp(rng.uniform(np.log(1e-9), np.log(0.0625), 4000)),  # quick phase
-        rng.uniform(0.02, 0.0625, 2000),                     # both quick-phase polynomials
+        rng.uniform(0.02, 0.0625, 2000),                     # the tight quick-phase tiers
+        rng.uniform(0.0625, 0.25, 4000),                     # the wide quick-phase tier
         rng.uniform(0.0625, 0.8, 2000),                      # accurate phase, no reduction
         rng.uniform(0.8, 7.0, 2000),                          # one to four quadrants
         np.exp(rng.uniform(np.log(7.0), np.log(1e6), 1000)),  # long reductions
         np.array([k * math.pi / 2 for k in range(1, 40)]),   # near multiples of pi/2
         np.array([2.0**-27, 2.0**-26, 2.0**-4, np.nextafter(2.0**-4, 1), np.nextafter(2.0**-4, 0), 2.0**-5,
-                  np.nextafter(2.0**-5, 1), np.nextafter(2.0**-5, 0), 0.785398163397448,
+                  np.nextafter(2.0**-5, 1), np.nextafter(2.0**-5, 0), 0.25, np.nextafter(0.25, 1),
+                  np.nextafter(0.25, 0), 0.785398163397448,
                   0.7853981633974484, 1e-300, 5e-324]),
     ]
     x = np.concatenate(parts)
