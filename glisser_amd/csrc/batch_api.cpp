@@ -46,7 +46,30 @@ constexpr double kHistSlack = 1e-8;             // allowed overrun of a history'
 struct SystemHost {
     uint32_t nb, n;
     double t_first, t_last;
+    // device slot k holds the caller's particle order[k] (set by upload)
+    std::vector<uint32_t> order;
 };
+
+// Device order of a system's particles: by 1/a = 2/|r| - v^2/mu, largest
+// first (shortest periods, hence largest mean-anomaly steps, first), ties
+// and orbits that are not bound (1/a <= 0 or NaN) in the caller's order at
+// the end.  A wave then holds particles of similar step size, so the waves
+// whose lanes need the long-step Kepler branch are few.  Results do not
+// depend on the order: every lane's arithmetic is its own.
+std::vector<uint32_t> frequency_order(uint32_t n, const double* r, const double* v, double mu) {
+    std::vector<double> key(n);
+    for (uint32_t k = 0; k < n; k++) {
+        const double* x = r + 3 * (size_t)k;
+        const double* u = v + 3 * (size_t)k;
+        const double ia = 2.0 / std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]) -
+                          (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / mu;
+        key[k] = ia > 0.0 ? ia : 0.0;  // NaN and unbound: 0
+    }
+    std::vector<uint32_t> order(n);
+    for (uint32_t k = 0; k < n; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return key[a] > key[b]; });
+    return order;
+}
 
 }  // namespace
 
@@ -411,12 +434,25 @@ int glx_batch_upload(glx_batch* b, uint32_t sys, const double* r, const double* 
     if (!r || !v || !flags || !id) return fail(GLX_EINVAL, "glx_batch_upload: r, v, flags, id are required");
     GLX_HIP(hipSetDevice(b->device));
     const uint32_t f = b->sys_h[sys].first;
+    std::vector<uint32_t>& order = b->host[sys].order;
+    order = frequency_order(n, r, v, b->sys_h[sys].mu);
+    std::vector<double> rv(3 * (size_t)n);
+    std::vector<uint16_t> fl(n);
+    std::vector<uint32_t> ids(n);
     for (int k = 0; k < 2; k++) {
-        GLX_HIP(hipMemcpyAsync(b->stage, k ? v : r, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, b->s));
+        const double* src = k ? v : r;
+        for (uint32_t j = 0; j < n; j++)
+            for (int c = 0; c < 3; c++) rv[3 * (size_t)j + c] = src[3 * (size_t)order[j] + c];
+        GLX_HIP(hipMemcpyAsync(b->stage, rv.data(), (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice, b->s));
         GLX_HIP(glx::launch_aos_to_soa(b->stage, b->d[3 * k] + f, b->d[3 * k + 1] + f, b->d[3 * k + 2] + f, n, b->s));
+        GLX_HIP(hipStreamSynchronize(b->s));  // rv is reused
     }
-    GLX_HIP(hipMemcpyAsync(b->flags + f, flags, (size_t)n * sizeof(uint16_t), hipMemcpyHostToDevice, b->s));
-    GLX_HIP(hipMemcpyAsync(b->id + f, id, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, b->s));
+    for (uint32_t j = 0; j < n; j++) {
+        fl[j] = flags[order[j]];
+        ids[j] = id[order[j]];
+    }
+    GLX_HIP(hipMemcpyAsync(b->flags + f, fl.data(), (size_t)n * sizeof(uint16_t), hipMemcpyHostToDevice, b->s));
+    GLX_HIP(hipMemcpyAsync(b->id + f, ids.data(), (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, b->s));
     GLX_HIP(hipMemsetAsync(b->dti + f, 0, (size_t)n * sizeof(uint32_t), b->s));
     GLX_HIP(hipStreamSynchronize(b->s));
     return GLX_OK;
@@ -430,17 +466,30 @@ int glx_batch_download(glx_batch* b, uint32_t sys, double* r, double* v, uint16_
     if (n == 0) return GLX_OK;
     GLX_HIP(hipSetDevice(b->device));
     const uint32_t f = b->sys_h[sys].first;
+    // slots back to the uploaded order (identity if nothing was uploaded)
+    const std::vector<uint32_t>& order = b->host[sys].order;
+    auto dst = [&](uint32_t j) { return order.empty() ? j : order[j]; };
     double* out[2] = {r, v};
+    std::vector<double> rv(3 * (size_t)n);
     for (int k = 0; k < 2; k++) {
         if (!out[k]) continue;
         GLX_HIP(glx::launch_soa_to_aos(b->d[3 * k] + f, b->d[3 * k + 1] + f, b->d[3 * k + 2] + f, b->stage, n, b->s));
-        GLX_HIP(hipMemcpyAsync(out[k], b->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, b->s));
+        GLX_HIP(hipMemcpyAsync(rv.data(), b->stage, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, b->s));
         GLX_HIP(hipStreamSynchronize(b->s));
+        for (uint32_t j = 0; j < n; j++)
+            for (int c = 0; c < 3; c++) out[k][3 * (size_t)dst(j) + c] = rv[3 * (size_t)j + c];
     }
-    if (flags) GLX_HIP(hipMemcpyAsync(flags, b->flags + f, (size_t)n * sizeof(uint16_t), hipMemcpyDeviceToHost, b->s));
-    if (id) GLX_HIP(hipMemcpyAsync(id, b->id + f, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, b->s));
-    if (dti) GLX_HIP(hipMemcpyAsync(dti, b->dti + f, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, b->s));
+    std::vector<uint16_t> fl(flags ? n : 0);
+    std::vector<uint32_t> ids(id ? n : 0), dt(dti ? n : 0);
+    if (flags) GLX_HIP(hipMemcpyAsync(fl.data(), b->flags + f, (size_t)n * sizeof(uint16_t), hipMemcpyDeviceToHost, b->s));
+    if (id) GLX_HIP(hipMemcpyAsync(ids.data(), b->id + f, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, b->s));
+    if (dti) GLX_HIP(hipMemcpyAsync(dt.data(), b->dti + f, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, b->s));
     GLX_HIP(hipStreamSynchronize(b->s));
+    for (uint32_t j = 0; j < n; j++) {
+        if (flags) flags[dst(j)] = fl[j];
+        if (id) id[dst(j)] = ids[j];
+        if (dti) dti[dst(j)] = dt[j];
+    }
     return GLX_OK;
 }
 

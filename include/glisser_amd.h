@@ -311,7 +311,10 @@ int glx_batch_destroy(glx_batch* b);
 
 /* Particles of system `sys` (its n_particles entries; r, v: n x 3), synchronous.
  * On download any output may be NULL; deathtime_index counts the steps since
- * the batch started (flagged particles keep the step count of their last step). */
+ * the batch started (flagged particles keep the step count of their last step).
+ * The device keeps a system's particles ordered by orbital frequency (so the
+ * lanes of a wave take similar Kepler steps); download returns them in the
+ * order of the upload. */
 int glx_batch_upload(glx_batch* b, uint32_t sys, const double* r, const double* v, const uint16_t* flags,
                      const uint32_t* id);
 int glx_batch_download(glx_batch* b, uint32_t sys, double* r, double* v, uint16_t* flags, uint32_t* id,
