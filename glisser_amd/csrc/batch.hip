@@ -39,8 +39,16 @@ __global__ void batch_finish_kernel(uint32_t* clock, uint32_t* queue, uint32_t s
     }
 }
 
-template <int FORM>
-__device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, double* lds) {
+// Bodies up to which a specialised tile keeps the GM and squared radii in
+// vector registers (more would cost the kernel its third wave per SIMD).
+constexpr uint32_t kBatchRegBodies = 4;
+
+// One tile.  NB > 0: the system's body count as a compile-time constant
+// (planet loop unrolled without guards; up to kBatchRegBodies bodies the GM
+// and squared radii in vector registers); NB == 0: any count up to
+// kBatchMaxBodies.
+template <int FORM, int NB>
+__device__ __forceinline__ void run_tile_nb(const BatchArgs& B, uint32_t tile, double* lds) {
     const uint32_t si = B.tile_sys[tile];
     const SystemDev& S = B.sys[si];
     const uint32_t i = B.tile_first[tile] + threadIdx.x;
@@ -71,6 +79,22 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
     }
     if constexpr (FORM != GLX_FORM_FP64) radius_from_position(p);
 
+    constexpr bool kInRegs = NB > 0 && NB <= (int)kBatchRegBodies;
+    constexpr int kH = kInRegs ? NB : 1;
+    double hm[kH], hrh2[kH];
+    const double* pm = S.pc.m;
+    const double* prh2 = S.pc.rh2;
+    if constexpr (kInRegs) {
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            hm[j] = S.pc.m[j];
+            hrh2[j] = S.pc.rh2[j];
+            pin(hm[j]);
+            pin(hrh2[j]);
+        }
+        pm = hm;
+        prh2 = hrh2;
+    }
     TaylorLead lead = taylor_lead();
     if constexpr (FORM != GLX_FORM_FP64) {
         pin(lead.s7);
@@ -84,8 +108,8 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
         const uint32_t total = cn * row;
         for (uint32_t k = threadIdx.x; k < total; k += blockDim.x) lds[k] = src[k];
         __syncthreads();
-        chunk_steps<0, FORM, kBatchMaxBodies>(p, dti, S, S.pc.m, S.pc.rh2, lds, row, cn, nb, base + cs, base + cs,
-                                              pending, lead);
+        chunk_steps<NB, FORM, (NB > 0 ? NB : (int)kBatchMaxBodies)>(p, dti, S, pm, prh2, lds, row, cn, nb, base + cs,
+                                                             base + cs, pending, lead);
     }
     if constexpr (FORM != GLX_FORM_FP64) close_kick(p, S.half_dt, pending);
 
@@ -102,6 +126,24 @@ __device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, doub
         B.flags[i] = (uint16_t)p.flags;
         B.dti[i] = dti;
     }
+}
+
+// FAST / MIXED tiles of systems with 2 to 4 bodies run a copy specialised
+// for their body count (the tile's system, hence the branch, is the same
+// for the whole workgroup); more bodies, and the fp64 form, run the generic
+// copy.  A copy for 5 or more bodies would take the kernel past the
+// registers of three waves per SIMD, whichever tiles it serves.
+template <int FORM>
+__device__ __forceinline__ void run_tile(const BatchArgs& B, uint32_t tile, double* lds) {
+    if constexpr (FORM != GLX_FORM_FP64) {
+        switch (B.sys[B.tile_sys[tile]].nb) {
+            case 2: return run_tile_nb<FORM, 2>(B, tile, lds);
+            case 3: return run_tile_nb<FORM, 3>(B, tile, lds);
+            case 4: return run_tile_nb<FORM, 4>(B, tile, lds);
+            default: break;
+        }
+    }
+    run_tile_nb<FORM, 0>(B, tile, lds);
 }
 
 template <int FORM>

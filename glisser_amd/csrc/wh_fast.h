@@ -374,6 +374,11 @@ __device__ __forceinline__ void kick_fma(Particle& p, double half_dt) {
     p.vz = fma(p.az, half_dt, p.vz);
 }
 
+// Bodies up to which a FAST / MIXED step copies its planet row from LDS into
+// registers ahead of the drift (3 doubles per body); with more bodies the
+// copy would cost more registers than the read latency it hides.
+constexpr int kRowRegBodies = 5;
+
 // One FAST / MIXED step of every active lane (none of them flagged).  C:
 // the launch's constants (dt, half_dt, mu, inv_mu, sqrt_mu, inv_sqrt_mu,
 // pc, outer2); hm, hrh2: the bodies' GM and squared radii as the planet sum
@@ -393,7 +398,7 @@ __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A
                                           const double* hrh2, const double* rowp, uint32_t nb, uint32_t death,
                                           double kick_in, const TaylorLead& lead) {
     uint64_t event;
-    if constexpr (NB > 0) {
+    if constexpr (NB > 0 && NB <= kRowRegBodies) {
         // the step's planet row is read from LDS before the drift, so the
         // read latency hides behind it
         double rowv[3 * NB];
