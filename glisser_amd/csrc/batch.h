@@ -30,8 +30,9 @@ struct SystemDev {
     double half_dt;
     double mu;
     double inv_mu;
-    double sqrt_mu;
+    double dt_sqrt_mu;   // dt sqrt(mu)
     double inv_sqrt_mu;
+    double inv_dt;
     double outer2;
     double sun_vote;     // carried |r| below this or above outer_vote
     double outer_vote;   // triggers the exact Sun / outer tests

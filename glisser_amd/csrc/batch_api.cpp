@@ -401,8 +401,9 @@ int glx_batch_create(int device, const glx_system* systems, uint32_t n_systems, 
         S.half_dt = y.dt / 2;
         S.mu = y.m[0];
         S.inv_mu = 1.0 / y.m[0];
-        S.sqrt_mu = std::sqrt(y.m[0]);
-        S.inv_sqrt_mu = 1.0 / S.sqrt_mu;
+        S.dt_sqrt_mu = y.dt * std::sqrt(y.m[0]);
+        S.inv_sqrt_mu = 1.0 / std::sqrt(y.m[0]);
+        S.inv_dt = 1.0 / y.dt;
         S.outer2 = y.outer_radius * y.outer_radius;
         S.sun_vote = std::sqrt(S.pc.rh2[0]) * (1.0 + glx::kVoteMargin);
         S.outer_vote = y.outer_radius * (1.0 - glx::kVoteMargin);

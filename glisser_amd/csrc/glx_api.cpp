@@ -444,8 +444,9 @@ BlockArgs block_args(glx_ctx* c, const PlanetBuf& b, double dt, double outer_rad
     A.half_dt = dt / 2;
     A.mu = b.mu;
     A.inv_mu = 1.0 / b.mu;
-    A.sqrt_mu = std::sqrt(b.mu);
-    A.inv_sqrt_mu = 1.0 / A.sqrt_mu;
+    A.dt_sqrt_mu = dt * std::sqrt(b.mu);
+    A.inv_sqrt_mu = 1.0 / std::sqrt(b.mu);
+    A.inv_dt = 1.0 / dt;
     A.outer2 = outer_radius * outer_radius;
     A.sun_vote = std::sqrt(b.pc.rh2[0]) * (1.0 + glx::kVoteMargin);
     A.outer_vote = outer_radius * (1.0 - glx::kVoteMargin);

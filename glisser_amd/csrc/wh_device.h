@@ -74,9 +74,10 @@ struct BlockArgs {
     double half_dt;
     double mu;
     double outer2;
-    double inv_mu;       // 1/mu, sqrt(mu), 1/sqrt(mu): FAST / MIXED drift
-    double sqrt_mu;
+    double inv_mu;       // 1/mu, dt sqrt(mu), 1/sqrt(mu), 1/dt: FAST / MIXED drift
+    double dt_sqrt_mu;
     double inv_sqrt_mu;
+    double inv_dt;
     double sun_vote;     // FAST / MIXED: carried |r| below this or above
     double outer_vote;   // this triggers the exact Sun / outer tests
     PlanetConsts pc;

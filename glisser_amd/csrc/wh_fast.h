@@ -104,12 +104,14 @@ struct TaylorLead {
 };
 __device__ __forceinline__ TaylorLead taylor_lead() { return {kTaylor[0][2], kTaylor[1][2]}; }
 
-// sincos_taylor<3> with the highest coefficients from `lead`.
-__device__ __forceinline__ void sincos_taylor3(double x, const TaylorLead& lead, double& s, double& cm1) {
+// sincos_taylor<3> with the highest coefficients from `lead`, returning
+// sin x - x (smx, whose rounding then stays relative to x^3 / 6) instead of
+// sin x.
+__device__ __forceinline__ void sincos_taylor3(double x, const TaylorLead& lead, double& smx, double& cm1) {
     const double x2 = x * x;
     const double ps = fma(fma(lead.s7, x2, kTaylor[0][1]), x2, kTaylor[0][0]);
     const double pc = fma(fma(lead.c6, x2, kTaylor[1][1]), x2, kTaylor[1][0]);
-    s = fma(ps * x2, x, x);
+    smx = (ps * x2) * x;
     cm1 = pc * x2;
 }
 
@@ -181,7 +183,8 @@ __device__ __forceinline__ void radius_from_position(Particle& p) {
 // Returns the wave mask of those lanes (the only ones whose flags the drift
 // can set).
 __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu, double inv_mu,
-                                               double sqrt_mu, double inv_sqrt_mu, const TaylorLead& lead) {
+                                               double dt_sqrt_mu, double inv_sqrt_mu, double inv_dt,
+                                               const TaylorLead& lead) {
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
     const double vx0 = p.vx, vy0 = p.vy, vz0 = p.vz;
     const double v2 = fma(vz0, vz0, fma(vy0, vy0, vx0 * vx0));
@@ -192,15 +195,14 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     const double rs_ia = rsqrt_nr1(ia);            // sqrt(a)
     const double a = rs_ia * rs_ia;
     const double sq_ia = ia * rs_ia;               // sqrt(1/a)
-    const double n_ = sqrt_mu * ia * sq_ia;        // sqrt(mu / a^3)
-    const double in_ = inv_sqrt_mu * a * rs_ia;    // 1/n
+    const double in_ = inv_sqrt_mu * a * rs_ia;    // 1/n, n = sqrt(mu / a^3)
     const double omec = p.rl * ia;                 // 1 - e cos E0 = |r| / a
     const double ecosEo = 1.0 - omec;
     const double esinEo = vdotr * (inv_sqrt_mu * sq_ia);  // r.v / (n a^2)
     const double q = a * idist;                    // a / |r| = 1 / (1 - e cos E0)
     // |dM| <= 0.2 on this path, so no full turns are removed and the time
     // left to advance, dM / n, is dt itself
-    const double dM = dt * n_;
+    const double dM = dt_sqrt_mu * ia * sq_ia;     // n dt
     // Kepler residual F(E) = E - ecosEo sin E + esinEo (1 - cos E) - dM and
     // F'(E) = 1 - ecosEo cos E + esinEo sin E, written with dd = E - dM and
     // (s, cm1) = (sin E, cos E - 1).  Start: the series of F in E,
@@ -211,25 +213,23 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     // (below 3e-11 at t = 0.008, e = 0.1), so one correction usually ends
     // the solve; any error of the start is corrected by the Newton steps and
     // only decides how many run.
-    const double cq = ecosEo * q;
     const double h = esinEo * q;
     const double A2 = 0.5 * h;
-    const double b3 = fma(A2, h, cq * (-1.0 / 6.0));  // 2 A2^2 - A3
+    // 2 A2^2 - A3, with -A3 = -c q / 6 = (1 - q) / 6 (c q = q - 1, as
+    // (1 - c) q = 1): one fused operation on the coefficient -1/6
+    const double b3 = fma(A2, h, fma(q, kTaylor[0][0], -kTaylor[0][0]));
     const double t = dM * q;
     const double tu = t * fma(t, b3, -A2);
     const double E1 = fma(tu, t, t);
     // exact (Sterbenz) while E1 / dM lies in [1/2, 2], which holds for
     // e < 1/3; otherwise its rounding is one more part of the start's error
     double dd = E1 - dM;
-    // lanes for the reference-arithmetic drift, as a wave mask: an unbound
-    // orbit (1/a <= 0) makes sqrt(1/a), hence n and dM, NaN, so one compare
-    // also catches it
-    uint64_t bad = wave_mask(!(fabs(dM) <= kMaxDM));
     // sin / cos - 1 of the start: three Taylor terms suffice for |E1| <= 0.02
     // (the regime of the long-period swarms this path is built for); a lane
     // with a larger start takes the retry below, which evaluates five terms
-    double s, cm1;
-    sincos_taylor3(E1, lead, s, cm1);
+    double smE, cm1;
+    sincos_taylor3(E1, lead, smE, cm1);
+    double s = E1 + smE;
     // First correction, short form: raw reciprocal and the rotation
     // sin(E + d) = s + d (1 + cm1), cos(E + d) - 1 = cm1 - s d, accepted only
     // within kShortD, kShortFp.  A Newton correction d leaves an error of at most
@@ -238,30 +238,40 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     double fp = fma(esinEo, s, fma(-ecosEo, cm1, omec));
     double r = __builtin_amdgcn_rcp(fp);
     const double d = -f * r;
-    dd += d;
+    // sin E - E at the corrected E: (s + d + cm1 d) - (E1 + d)
+    smE = fma(cm1, d, smE);
     const double s1 = fma(cm1, d, s + d);
     cm1 = fma(-s, d, cm1);
     s = s1;
+    // A lane inside these bounds needs no test of |dM|: its corrected E is
+    // the Kepler equation's root to within 1e-20 and |E| <= 0.02 + 7e-11,
+    // so |dM| = |E - ecosEo sin E + esinEo (1 - cos E)| <= 0.0402 (e < 1
+    // for 1/a > 0; an unbound orbit makes every quantity here NaN, which
+    // fails the first test).
     const uint64_t retry =
         wave_mask(!(fabs(d) < kShortD)) | wave_mask(!(fp > kShortFp)) | wave_mask(!(fabs(E1) <= kMaxStart));
     // Lanes outside that bound (large steps or eccentric orbits) restart from
     // sin / cos of the corrected E and take full corrections (accurate
     // reciprocal, third-order rotation, the reference's 1e-14 tolerance as a
     // proven bound); these run only in waves where some lane needs them.
+    // Lanes for the reference-arithmetic drift (bad): |dM| or the corrected
+    // |E| above kMaxDM, or no proven convergence.
+    uint64_t bad = 0;
     if (retry != 0) {
         bool conv = true, far = false;
         if (lane_in(retry)) {
+            dd += d;
             const double E2 = dM + dd;
-            far = !(fabs(E2) <= kMaxDM);
+            far = !(fabs(E2) <= kMaxDM) || !(fabs(dM) <= kMaxDM);
             sincos_taylor<5>(E2, s, cm1);
             conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1, r);
         }
         if (wave_any(!conv)) {
             if (!conv) conv = newton_step(esinEo, ecosEo, omec, dd, s, cm1, r);
         }
-        bad |= wave_mask(far) | wave_mask(!conv);
+        if (lane_in(retry)) smE = s - (dM + dd);
+        bad = wave_mask(far) | wave_mask(!conv);
     }
-    const double dE = dM + dd;
     // F'(E) - (1 - e cos E0) = esinEo sin dE - ecosEo (cos dE - 1)
     const double dfp = fma(esinEo, s, -ecosEo * cm1);
     fp = omec + dfp;
@@ -271,8 +281,8 @@ __device__ __forceinline__ uint64_t drift_fast(Particle& p, double dt, double mu
     p.rl = fma(a, dfp, p.rl);
     p.ir = ia * ifp;
     double fg = fma(cm1, q, 1.0);
-    double g = fma(s - dE, in_, dt);
-    double fdot = -(n_ * q) * s * ifp;
+    double g = fma(smE, in_, dt);
+    double fdot = -(t * inv_dt) * s * ifp;         // n q = t / dt
     double gdot = fma(cm1, ifp, 1.0);
     // Lanes outside the shortcuts take the f and g functions of the
     // reference-arithmetic drift (and its 0x04 / 0x08 flags) instead; the
@@ -405,12 +415,12 @@ __device__ __forceinline__ bool fast_step(Particle& p, uint32_t& dti, const C& A
 #pragma unroll
         for (int k = 0; k < 3 * NB; k++) rowv[k] = rowp[k];
         kick_fma(p, kick_in);
-        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu, lead);
+        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.dt_sqrt_mu, A.inv_sqrt_mu, A.inv_dt, lead);
         event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowv, A.pc, hm, hrh2, A.sun_vote, A.outer_vote,
                                                               A.outer2, nb);
     } else {
         kick_fma(p, kick_in);
-        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.sqrt_mu, A.inv_sqrt_mu, lead);
+        event = drift_fast(p, A.dt, A.mu, A.inv_mu, A.dt_sqrt_mu, A.inv_sqrt_mu, A.inv_dt, lead);
         event |= accel_vote<NB, FORM == GLX_FORM_MIXED, NMAX>(p, rowp, A.pc, hm, hrh2, A.sun_vote, A.outer_vote,
                                                               A.outer2, nb);
     }
