@@ -489,11 +489,18 @@ __device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C&
     if constexpr (FORM != GLX_FORM_FP64) {
         if (wave_mask(p.flags != 0) == 0) {
             while (s + 1 < cn) {
+                // the pair's row offset held in a vector register: both
+                // steps read their rows from it (the second through the
+                // LDS instruction's offset field), one register move per
+                // pair instead of one per step
+                uint32_t off = s * row;
+                __asm__ volatile("" : "+v"(off));
+                const double* pair = rows + off;
                 if (((g0 + s) & kMask) == 0) {
                     close_kick(p, A.half_dt, pending);
                     radius_from_position(p);
                 }
-                bool flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1,
+                bool flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, pair, nb, death0 + s + 1,
                                                          pending ? A.dt : A.half_dt, lead);
                 pending = true;
                 s++;
@@ -502,7 +509,7 @@ __device__ __forceinline__ void chunk_steps(Particle& p, uint32_t& dti, const C&
                     close_kick(p, A.half_dt, pending);
                     radius_from_position(p);
                 }
-                flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, rows + s * row, nb, death0 + s + 1,
+                flagged = fast_step<NB, FORM, NMAX>(p, dti, A, hm, hrh2, pair + row, nb, death0 + s + 1,
                                                     pending ? A.dt : A.half_dt, lead);
                 pending = true;
                 s++;
