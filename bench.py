@@ -58,14 +58,26 @@ HEADLINE_FORM = "mixed"
 # summed over the block kernel and divided by its particle-steps); see
 # profiles/wh_block_200k_counters.md.  Used with the live rate to report
 # the counter-based share of FP64 vector peak.
-COUNTER_FLOPS_PER_PSTEP = {"fp64": 979.0, "fast": 230.0, "mixed": 226.0}
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 948.25, "fast": 210.34, "mixed": 206.34}
+
+# VALU-busy SIMD cycles per particle-step (rocprofv3 SQ_ACTIVE_INST_VALU, in
+# quad-cycles, x 4 / particle-steps, one 200k-particle block in one
+# dispatch) and the shader clock during the kernel (GRBM_GUI_ACTIVE summed
+# over the 8 XCDs / 8 / dispatch duration); see
+# profiles/wh_block_200k_counters.md.  Used with the live rate to report the
+# share of the SIMDs' issue cycles the kernel keeps busy and the roofline
+# fraction at the measured clock.
+VALU_CYCLES_PER_PSTEP = {"fp64": 66.23, "fast": 10.545, "mixed": 10.545}
+CLOCK_GHZ_MEASURED = {"fp64": 2.274, "fast": 2.207, "mixed": 2.245}
+N_SIMDS = 1024  # 256 CUs x 4 SIMDs
+PEAK_CLOCK_GHZ = 2.4
 
 # HBM bytes per time-block of the mixed form at this workload and launch
 # shape (4 slices x 4 launches per block: each launch reads and writes its
 # slice's state once), from rocprofv3 FETCH_SIZE (x 2.0, the factor a copy
 # kernel of known size needs) and WRITE_SIZE (x 1.0); see
 # profiles/wh_block_200k_counters.md.
-TRAFFIC_BYTES_PER_BLOCK = {"mixed": 133.1e6}
+TRAFFIC_BYTES_PER_BLOCK = {"mixed": 133.3e6}
 
 TRACK_RECORD_BYTES = 28  # u32 id + 6 float32 per particle in a track snapshot
 
@@ -123,7 +135,7 @@ def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int) -> int:
 # kernel is the profiled one.
 KERNEL_SOURCES = ["glisser_amd/csrc/wh_kernels.hip", "glisser_amd/csrc/wh_fast.h", "glisser_amd/csrc/wh_device.h",
                   "glisser_amd/csrc/sincos_cr.h", "glisser_amd/csrc/launch.h"]
-PROFILED_KERNEL_SHA256 = ""
+PROFILED_KERNEL_SHA256 = "c03cb9067d8ff0a8083b3140e354da65c06612876de26a4275f57169f7b56136"
 
 
 def kernel_fingerprint() -> str:
@@ -704,6 +716,9 @@ def main(argv=None):
         cf = COUNTER_FLOPS_PER_PSTEP.get(form)
         if cf and counters_current:
             entry["counter_fp64_frac"] = per_block * cf / block_s / 1e12 / FP64_PEAK_TFLOPS
+            clk = CLOCK_GHZ_MEASURED[form]
+            entry["valu_busy"] = per_block * VALU_CYCLES_PER_PSTEP[form] / block_s / (N_SIMDS * clk * 1e9)
+            entry["frac_at_measured_clock"] = entry["alg_tflops"] / (FP64_PEAK_TFLOPS * clk / PEAK_CLOCK_GHZ)
         per_form[form] = entry
     h = per_form[head]
     line = {
@@ -750,6 +765,9 @@ def main(argv=None):
                         else None),
             "traffic_note": "HBM bytes per launch of `steps` blocks (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
             "counter_fp64_frac": h.get("counter_fp64_frac"),
+            "clock_ghz_measured": CLOCK_GHZ_MEASURED[head] if counters_current else None,
+            "frac_at_measured_clock": h.get("frac_at_measured_clock"),
+            "valu_busy": h.get("valu_busy"),
             "counters_note": ("from the counters of profiles/wh_block_200k_counters.md (same kernel sources)"
                               if counters_current else
                               "kernel sources differ from the profiled build: counter-based fields omitted"),
