@@ -34,6 +34,7 @@ struct SystemDev {
     double inv_sqrt_mu;
     double inv_dt;
     double outer2;
+    uint32_t fp64_core;  // fp64 form: fp64_core_constants(mu, dt)
     double sun_vote;     // carried |r| below this or above outer_vote
     double outer_vote;   // triggers the exact Sun / outer tests
     uint32_t nb;         // bodies (Sun + planets)

@@ -405,6 +405,7 @@ int glx_batch_create(int device, const glx_system* systems, uint32_t n_systems, 
         S.inv_sqrt_mu = 1.0 / std::sqrt(y.m[0]);
         S.inv_dt = 1.0 / y.dt;
         S.outer2 = y.outer_radius * y.outer_radius;
+        S.fp64_core = glx::fp64_core_constants(y.m[0], y.dt) ? 1u : 0u;
         S.sun_vote = std::sqrt(S.pc.rh2[0]) * (1.0 + glx::kVoteMargin);
         S.outer_vote = y.outer_radius * (1.0 - glx::kVoteMargin);
         S.nb = y.n_bodies;

@@ -448,6 +448,7 @@ BlockArgs block_args(glx_ctx* c, const PlanetBuf& b, double dt, double outer_rad
     A.inv_sqrt_mu = 1.0 / std::sqrt(b.mu);
     A.inv_dt = 1.0 / dt;
     A.outer2 = outer_radius * outer_radius;
+    A.fp64_core = glx::fp64_core_constants(b.mu, dt) ? 1u : 0u;
     A.sun_vote = std::sqrt(b.pc.rh2[0]) * (1.0 + glx::kVoteMargin);
     A.outer_vote = outer_radius * (1.0 - glx::kVoteMargin);
     A.block_step0 = 0;

@@ -74,6 +74,7 @@ struct BlockArgs {
     double half_dt;
     double mu;
     double outer2;
+    uint32_t fp64_core;  // fp64 form: fp64_core_constants(mu, dt)
     double inv_mu;       // 1/mu, dt sqrt(mu), 1/sqrt(mu), 1/dt: FAST / MIXED drift
     double dt_sqrt_mu;
     double inv_sqrt_mu;
@@ -88,6 +89,62 @@ struct Particle {
     double rl, ir;  // |r| and 1/|r| carried between steps (FAST / MIXED forms only)
     uint32_t flags;
 };
+
+// Correctly rounded FP64 divide and square root (round to nearest) for
+// operands in a range where the compiler's IEEE expansions rescale and fix
+// up nothing.  For gfx950 hipcc expands a / b into v_div_scale_f64 of b and
+// of a, r = v_rcp_f64(b), two Newton steps r = r + r (1 - b r),
+// q = a r, q = q + (a - b q) r (v_div_fmas_f64) and v_div_fixup_f64, and
+// sqrt(x) into a range test with v_ldexp_f64, y = v_rsq_f64(x), the
+// Goldschmidt iteration below and a class test for zero and infinity.  The
+// scaling steps change nothing while a, b, a / b and 1 / b are normal, |a| >=
+// 2^-969 and the exponents of a and b differ by less than 768; the fix-up
+// changes nothing for finite nonzero normal results; the square root's
+// rescaling and class test change nothing for normal x >= 2^-767.  Inside
+// those bounds the cores below are the same operations, so they return
+// the same correctly rounded results with 3 (divide) and 8 (square root)
+// instructions fewer.  The fp64 form uses them only where guards on the
+// particle's state bound every operand well inside (kepler_fg_fp64,
+// accel_fp64) and recomputes any other lane with the IEEE operators.
+
+// a / b for b > 0 and a = 0 or a, b, a / b normal inside the bounds above
+// (used for |a| in [2^-400, 2^300], b in [2^-300, 2^300]).  The expansion is
+// run on -a and its result negated (round to nearest is symmetric), which
+// also gives the IEEE sign of a zero quotient for either sign of a zero a.
+__device__ __forceinline__ double core_div_pos(double a, double b) {
+#pragma clang fp contract(off)
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    const double q = (-a) * r;
+    const double rem = fma(-b, q, -a);
+    return fma(-rem, r, -q);
+}
+
+// sqrt(x) for normal x inside the bounds above (used for x in [2^-700,
+// 2^700]).
+__device__ __forceinline__ double core_sqrt(double x) {
+#pragma clang fp contract(off)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    double d = fma(-g, g, x);
+    h = fma(h, r, h);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
+// Whether a launch's constants admit the guarded cores: mu (GM of the
+// central body) and dt in [2^-100, 2^100].  Every bound in the guards of
+// kepler_fg_fp64 and accel_fp64 starts from these.
+__host__ __device__ inline bool fp64_core_constants(double mu, double dt) {
+    return mu >= 0x1p-100 && mu <= 0x1p100 && dt >= 0x1p-100 && dt <= 0x1p100;
+}
 
 // f and g functions of the Kepler drift in the reference's arithmetic
 // (src/wh.cu:74-110 with kepeq at src/wh.cu:42-72): fixed five Newton
@@ -104,24 +161,57 @@ struct Particle {
 // throws).  kCrTrig false: sin/cos from the device math library instead
 // (the FAST / MIXED forms' fallback lanes, which need no bit-equality with
 // the CPU path and keep their registers for the common path).
-template <bool kCrTrig = true>
+//
+// kCore: the divides and square roots whose operands the guards below bound
+// run as core_div_pos / core_sqrt (same results there); ok is cleared for a
+// lane outside the guards, whose results are then to be recomputed with
+// kCore false.  The guards, given mu and dt in [2^-100, 2^100]
+// (fp64_core_constants): |r|^2 in [2^-180, 2^200] and |v|^2 <= 2^200, so
+// |r| in [2^-90, 2^100], mu / |r| in [2^-200, 2^190] and a nonzero energy
+// (a difference of doubles of at least 2^-201) in [2^-253, 2^200]; then
+// a = 0.5 mu / -energy in [|r| / 2, 2^352], guarded a <= 2^100, so a^3 in
+// [2^-274, 2^300], mu / a^3 in [2^-400, 2^374], n in [2^-200, 2^187],
+// |r| / a in [2^-191, 2], dM = dt n (no turns removed; the rare other
+// branch clears ok) in [2^-300, 1.5) and dM / n near dt.  Each Newton
+// denominator F' is guarded >= 2^-50, which keeps |dE| below 2^254 over
+// five iterations (|F| <= |dE| + 5); a nonzero residual F = X - dM is then
+// at least 2^-353 (Sterbenz, or larger than dM / 2).  sin dE - dE is 0 (sin
+// rounds to dE below 2^-27) or at least 2^-80, cos dE - 1 is 0 or at least
+// 2^-54 in magnitude, and the final F' is guarded >= 2^-50.  e sin E0 =
+// r.v / (n a^2) (r.v may be arbitrarily small) and fdot's quotient (sin dE
+// may be) keep the IEEE divide.
+template <bool kCrTrig = true, bool kCore = false>
 __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double rz, double vx, double vy,
                                                    double vz, double dt, double mu, double& f, double& g,
-                                                   double& fdot, double& gdot) {
+                                                   double& fdot, double& gdot, bool* ok = nullptr) {
 #pragma clang fp contract(off)
-    const double dist = sqrt(rx * rx + ry * ry + rz * rz);
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    double dist;
+    if constexpr (kCore) {
+        *ok = *ok && r2 >= 0x1p-180 && r2 <= 0x1p200 && v2 <= 0x1p200;
+        dist = core_sqrt(r2);
+    } else {
+        dist = sqrt(r2);
+    }
     const double vdotr = vx * rx + vy * ry + vz * rz;
-    const double energy = (vx * vx + vy * vy + vz * vz) * 0.5 - mu / dist;
+    const double energy = v2 * 0.5 - (kCore ? core_div_pos(mu, dist) : mu / dist);
     if (energy >= 0.0) {
-        f = 1.0;
-        g = 0.0;
-        fdot = 0.0;
-        gdot = 1.0;
+        // kCore (called by drift_fp64 only, which leaves an unbound orbit's
+        // state as it is): the identity f and g need not be set
+        if constexpr (!kCore) {
+            f = 1.0;
+            g = 0.0;
+            fdot = 0.0;
+            gdot = 1.0;
+        }
         return kFlagUnbound;
     }
-    const double a = -0.5 * mu / energy;
-    const double n_ = sqrt(mu / (a * a * a));
-    const double ecosEo = 1.0 - dist / a;
+    // -0.5 mu / energy: the same quotient as 0.5 mu / -energy
+    const double a = kCore ? core_div_pos(0.5 * mu, -energy) : -0.5 * mu / energy;
+    if constexpr (kCore) *ok = *ok && a <= 0x1p100;
+    const double n_ = kCore ? core_sqrt(core_div_pos(mu, a * a * a)) : sqrt(mu / (a * a * a));
+    const double ecosEo = 1.0 - (kCore ? core_div_pos(dist, a) : dist / a);
     const double esinEo = vdotr / (n_ * a * a);
     // dt*n - (pi*2)*(int)((dt*n/pi)*2): the reference's M_2PI macro
     // expansion.  For 0 <= dt*n < 1.5 the quotient lies below 0.955, the
@@ -129,8 +219,11 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
     // the divide only runs outside that range.
     const double dtn = dt * n_;
     double dM = dtn;
-    if (!(dtn >= 0.0 && dtn < 1.5)) dM = dtn - kPi * 2 * (double)(int)(dtn / kPi * 2);
-    const double adv_dt = dM / n_;
+    if (!(dtn >= 0.0 && dtn < 1.5)) {
+        dM = dtn - kPi * 2 * (double)(int)(dtn / kPi * 2);
+        if constexpr (kCore) *ok = false;
+    }
+    const double adv_dt = kCore ? core_div_pos(dM, n_) : dM / n_;
     double sM, cM;
     if constexpr (kCrTrig) cr::sincos_cr(dM, &sM, &cM);
     else sincos(dM, &sM, &cM);
@@ -143,25 +236,44 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
     for (int k = 0; k < kKepIts; k++) {
         const double fk = dE - ecosEo * sindE + esinEo * (1. - cosdE) - dM;
         const double fpk = 1. - ecosEo * cosdE + esinEo * sindE;
-        delta = -fk / fpk;
+        if constexpr (kCore) {
+            *ok = *ok && fpk >= 0x1p-50;
+            delta = core_div_pos(-fk, fpk);
+        } else {
+            delta = -fk / fpk;
+        }
         dE += delta;
         if constexpr (kCrTrig) cr::sincos_cr(dE, &sindE, &cosdE);
         else sincos(dE, &sindE, &cosdE);
     }
     const double fp = 1.0 - ecosEo * cosdE + esinEo * sindE;
-    f = 1.0 + a * (cosdE - 1.0) / dist;
-    g = adv_dt + (sindE - dE) / n_;
+    if constexpr (kCore) {
+        *ok = *ok && fp >= 0x1p-50;
+        f = 1.0 + core_div_pos(a * (cosdE - 1.0), dist);
+        g = adv_dt + core_div_pos(sindE - dE, n_);
+        gdot = 1.0 + core_div_pos(cosdE - 1.0, fp);
+    } else {
+        f = 1.0 + a * (cosdE - 1.0) / dist;
+        g = adv_dt + (sindE - dE) / n_;
+        gdot = 1.0 + (cosdE - 1.0) / fp;
+    }
     fdot = -n_ * sindE * a / (dist * fp);
-    gdot = 1.0 + (cosdE - 1.0) / fp;
     return fabs(delta) > kTolKep ? kFlagKepler : 0u;
 }
 
 // Kepler drift in the reference's arithmetic: kepler_fg_fp64 applied
-// without FMA contraction, as the reference does.
-__device__ __forceinline__ void drift_fp64(Particle& p, double dt, double mu) {
+// without FMA contraction, as the reference does.  core (wave-uniform:
+// fp64_core_constants of the launch): the guarded divide and square-root
+// cores, with the lanes outside their guards recomputed with the IEEE
+// operators (the same bits either way).
+__device__ __forceinline__ void drift_fp64(Particle& p, double dt, double mu, bool core) {
 #pragma clang fp contract(off)
     double f, g, fdot, gdot;
-    p.flags = kepler_fg_fp64(p.rx, p.ry, p.rz, p.vx, p.vy, p.vz, dt, mu, f, g, fdot, gdot);
+    bool ok = core;
+    uint32_t fl = 0;
+    if (core) fl = kepler_fg_fp64<true, true>(p.rx, p.ry, p.rz, p.vx, p.vy, p.vz, dt, mu, f, g, fdot, gdot, &ok);
+    if (!ok) fl = kepler_fg_fp64(p.rx, p.ry, p.rz, p.vx, p.vy, p.vz, dt, mu, f, g, fdot, gdot);
+    p.flags = fl;
     if (p.flags == kFlagUnbound) return;
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;
     p.rx = rx0 * f + p.vx * g;
@@ -183,10 +295,12 @@ __device__ __forceinline__ void kick(Particle& p, double half_dt) {
 // Acceleration from the step's planet row plus the encounter / cull /
 // outer-limit tests (src/wh.cu:142-170), in FP64 with IEEE divide/sqrt.
 // NB > 0: compile-time body count; NB == 0: nb bodies at run time, at most
-// NMAX.
-template <int NB, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES)>
+// NMAX.  kCore: each planet's 1 / (|dr|^2 sqrt(|dr|^2)) with core_sqrt and
+// core_div_pos, guarded |dr|^2 in [2^-200, 2^200] (so |dr|^3 in [2^-300,
+// 2^300]); ok is cleared for a lane outside the guard.
+template <int NB, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES), bool kCore = false>
 __device__ __forceinline__ void accel_fp64(Particle& p, const double* __restrict__ row, const PlanetConsts& pc,
-                                           double outer2, uint32_t nb) {
+                                           double outer2, uint32_t nb, bool* ok = nullptr) {
 #pragma clang fp contract(off)
     p.ax = row[0];
     p.ay = row[1];
@@ -200,7 +314,13 @@ __device__ __forceinline__ void accel_fp64(Particle& p, const double* __restrict
         const double dz = p.rz - row[3 * j + 2];
         const double rad = dx * dx + dy * dy + dz * dz;
         if (rad < pc.rh2[j] && p.flags == 0) p.flags = ((pc.id[j] << 8) | kFlagEncounter) & 0xFFFFu;
-        const double inv3 = 1. / (rad * sqrt(rad));
+        double inv3;
+        if constexpr (kCore) {
+            *ok = *ok && rad >= 0x1p-200 && rad <= 0x1p200;
+            inv3 = core_div_pos(1., rad * core_sqrt(rad));
+        } else {
+            inv3 = 1. / (rad * sqrt(rad));
+        }
         const double fac = pc.m[j] * inv3;
         p.ax -= dx * fac;
         p.ay -= dy * fac;
@@ -209,6 +329,21 @@ __device__ __forceinline__ void accel_fp64(Particle& p, const double* __restrict
     const double rad = p.rx * p.rx + p.ry * p.ry + p.rz * p.rz;
     if (rad < pc.rh2[0]) p.flags = kFlagEncounter;
     if (rad > outer2) p.flags = kFlagOut;
+}
+
+// accel_fp64 of one fp64 step: with the guarded cores when core (wave
+// uniform), a lane outside their guards recomputed with the IEEE operators
+// from its flags before the call (the same bits either way).
+template <int NB, int NMAX = (NB > 0 ? NB : GLX_MAX_BODIES)>
+__device__ __forceinline__ void accel_fp64_step(Particle& p, const double* __restrict__ row, const PlanetConsts& pc,
+                                                double outer2, uint32_t nb, bool core) {
+    const uint32_t flags0 = p.flags;
+    bool ok = core;
+    if (core) accel_fp64<NB, NMAX, true>(p, row, pc, outer2, nb, &ok);
+    if (!ok) {
+        p.flags = flags0;
+        accel_fp64<NB, NMAX>(p, row, pc, outer2, nb);
+    }
 }
 
 }  // namespace glx

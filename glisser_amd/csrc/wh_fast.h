@@ -444,8 +444,8 @@ __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const 
     if (p.flags != 0) return;
     if constexpr (FORM == GLX_FORM_FP64) {
         kick(p, A.half_dt);
-        drift_fp64(p, A.dt, A.mu);
-        accel_fp64<NB, NMAX>(p, rowp, A.pc, A.outer2, nb);
+        drift_fp64(p, A.dt, A.mu, A.fp64_core != 0);
+        accel_fp64_step<NB, NMAX>(p, rowp, A.pc, A.outer2, nb, A.fp64_core != 0);
         kick(p, A.half_dt);
         if (p.flags != 0) dti = death;
     } else {

@@ -34,7 +34,7 @@ def test_more_than_one_gpu_is_refused():
 
 def test_wave_cap_defaults(bench):
     auto = argparse.Namespace(waves_per_simd=None)
-    assert bench.waves_per_simd(auto, "fp64") == 3
+    assert bench.waves_per_simd(auto, "fp64") == 4
     assert bench.waves_per_simd(auto, "fast") == 0 and bench.waves_per_simd(auto, "mixed") == 0
     assert bench.waves_per_simd(argparse.Namespace(waves_per_simd=4), "mixed") == 4
 
