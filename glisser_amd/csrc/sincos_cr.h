@@ -125,10 +125,32 @@ constexpr double kQuickSinErr = 0x1p-70;  // relative to |x|
 constexpr double kQuickCosErr = 0x1p-69;  // absolute (cos x ~ 1)
 constexpr double kWideSinErr = 0x1p-64;   // the same for 2^-4 < |x| <= 2^-2
 constexpr double kWideCosErr = 0x1p-62;
+// Rounding-test factors of the quick phase for those bounds (sincos_quick).
+constexpr double kZivSin = 1.0 + 0x1p-15;
+constexpr double kZivCos = 1.0 + 0x1p-14;
+constexpr double kZivWideSin = 1.0 + 0x1p-9;
+constexpr double kZivWideCos = 1.0 + 0x1p-7;
+// 2^54 eps < 1 - 1 / z, with eps the error bound relative to the result
+// (|sin x| >= 0.9896 |x|, cos x >= 0.9689 for |x| <= 2^-2)
+static_assert(0x1p54 * kQuickSinErr / 0.9896 < 1.0 - 1.0 / kZivSin, "sin rounding test");
+static_assert(0x1p54 * kQuickCosErr / 0.9689 < 1.0 - 1.0 / kZivCos, "cos rounding test");
+static_assert(0x1p54 * kWideSinErr / 0.9896 < 1.0 - 1.0 / kZivWideSin, "wide sin rounding test");
+static_assert(0x1p54 * kWideCosErr / 0.9689 < 1.0 - 1.0 / kZivWideCos, "wide cos rounding test");
 // Accurate-phase error bounds: relative to the result (series and pair
 // arithmetic) plus the absolute error of the argument reduction.
 constexpr double kAccErr = 0x1p-98;
 constexpr double kRedErr = 0x1p-102;
+
+// Ziv's rounding test in its one-operation form: when the exact value y lies
+// within eps |hi| of hi + lo, RN(hi + lo z) == hi for z = 1 + 2^-k with
+// 2^54 eps < 1 - 1 / z proves RN(y) == hi.  (hi + lo z then lies in hi's
+// rounding interval, whose half-widths are at least 2^-54 |hi|, so |lo| is
+// at most that half-width / z, and |y - hi| <= half-width (1 / z + 2^54
+// eps) < half-width.)  Needs no normalisation of the pair.
+__host__ __device__ inline bool rounds_to_hi(double hi, double lo, double z) {
+#pragma clang fp contract(off)
+    return fma(lo, z, hi) == hi;
+}
 
 // RN(hi + lo) when the exact value lies within err of hi + lo and both ends
 // of that interval round alike; otherwise false.  hi + lo is normalised
@@ -159,10 +181,14 @@ __host__ __device__ inline bool round_decided(double hi, double lo, double err, 
 // s15 and c14: the same sums give |err| < 9 * 2^-53 * |x|^5 / 119 <
 // 2^-64.7 |x| for sin (dropped x^17 / 17! < 2^-80 |x|) and 7 * 2^-53 * x^4 /
 // 24 < 2^-62.8 for cos (dropped x^16 / 16! < 2^-76), so kWideSinErr =
-// 2^-64 |x| and kWideCosErr = 2^-62, which leave about one call in 200
-// undecided (2 err / ulp: 2^-10 for sin, 2^-8 for cos).  (A decided result is the correctly rounded value whichever
-// polynomial produced it, so the choice never changes the returned bits.)
-// Returns false when either result is undecided.
+// 2^-64 |x| and kWideCosErr = 2^-62.  (A decided result is the correctly
+// rounded value whichever polynomial produced it, so the choice never
+// changes the returned bits.)  Rounding test (rounds_to_hi): relative to
+// the result, |sin x| >= 0.9896 |x| and cos x >= 0.9689 on |x| <= 2^-2, so
+// eps <= 2^-69.98 (sin) and 2^-68.95 (cos), or 2^-63.98 and 2^-61.95 in the
+// wide tier; kZiv* = 1 + 2^-15, 1 + 2^-14, 1 + 2^-9 and 1 + 2^-7 satisfy
+// 2^54 eps < 1 - 1 / z, and leave about 2^-15, 2^-14, 2^-9 and 2^-7 of the
+// calls undecided.  Returns false when either result is undecided.
 __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
 #pragma clang fp contract(off)
     const double x2 = x * x;
@@ -173,7 +199,7 @@ __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
     const double t1 = kS3h * x3;
     const double t1l = fma(kS3h, x3, -t1) + fma(kS3h, x3l, kS3l * x3);
     const double ax = fabs(x);
-    double ps, pc, es = kQuickSinErr, ec = kQuickCosErr;
+    double ps, pc, zs = kZivSin, zc = kZivCos;
     if (ax <= kQuickShort) {
         ps = fma(fma(x2, kS9h, kS7h), x2, kS5h);
         pc = fma(fma(x2, kC8h, kC6h), x2, kC4h);
@@ -188,8 +214,8 @@ __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
         pc = fma(pc, x2, kC8h);
         pc = fma(pc, x2, kC6h);
         pc = fma(pc, x2, kC4h);
-        es = kWideSinErr;
-        ec = kWideCosErr;
+        zs = kZivWideSin;
+        zc = kZivWideCos;
     } else {
         ps = fma(x2, kS13h, kS11h);
         ps = fma(ps, x2, kS9h);
@@ -210,9 +236,9 @@ __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
     const Pair uh = fast_two_sum(h, u2);
     const Pair ch = fast_two_sum(1.0, uh.hi);
     const double clo = ch.lo + (uh.lo + hl);
-    const bool ok_s = round_decided(sh.hi, slo, es * ax, s);
-    const bool ok_c = round_decided(ch.hi, clo, ec, c);
-    return ok_s && ok_c;
+    s = sh.hi;
+    c = ch.hi;
+    return rounds_to_hi(sh.hi, slo, zs) & rounds_to_hi(ch.hi, clo, zc);
 }
 
 // Accurate phase for |x| < 2^20: reduction to y = x - k pi/2 as a pair,
