@@ -96,10 +96,7 @@ __device__ __forceinline__ void run_tile_nb(const BatchArgs& B, uint32_t tile, d
         prh2 = hrh2;
     }
     TaylorLead lead = taylor_lead();
-    if constexpr (FORM != GLX_FORM_FP64) {
-        pin(lead.s7);
-        pin(lead.c6);
-    }
+    pin_lead<FORM>(lead);
     bool pending = false;  // FAST / MIXED: closing half kick still due
     for (uint32_t cs = 0; cs < B.steps; cs += S.chunk) {
         const uint32_t cn = min(S.chunk, B.steps - cs);

@@ -189,7 +189,23 @@ __host__ __device__ inline bool round_decided(double hi, double lo, double err, 
 // wide tier; kZiv* = 1 + 2^-15, 1 + 2^-14, 1 + 2^-9 and 1 + 2^-7 satisfy
 // 2^54 eps < 1 - 1 / z, and leave about 2^-15, 2^-14, 2^-9 and 2^-7 of the
 // calls undecided.  Returns false when either result is undecided.
-__host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
+// The short tier's leading polynomial coefficients and rounding-test
+// factors, passed by value so a kernel can hold them in vector registers: a
+// fused multiply-add with two constant operands (or with a literal, which a
+// three-operand FP64 instruction cannot encode on this target) becomes a
+// register copy plus the two-operand form, whose accumulator is the
+// constant.
+struct QuickLead {
+    double s9, c8, zs, zc;
+};
+constexpr QuickLead kQuickLead = {kS9h, kC8h, kZivSin, kZivCos};
+
+// One tier of the quick phase (0: |x| <= 2^-5, 1: |x| <= 2^-4, 2: |x| <=
+// 2^-2), evaluated straight through: each tier is its own code path, so its
+// polynomial coefficients and rounding-test factors stay constants (tier 0:
+// those of L, the values of kQuickLead).
+template <int kTier>
+__host__ __device__ inline bool sincos_quick_tier(double x, double& s, double& c, const QuickLead& L) {
 #pragma clang fp contract(off)
     const double x2 = x * x;
     const double x2l = fma(x, x, -x2);
@@ -198,12 +214,20 @@ __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
     const double x3l = fma(x, x2, -x3) + x * x2l;
     const double t1 = kS3h * x3;
     const double t1l = fma(kS3h, x3, -t1) + fma(kS3h, x3l, kS3l * x3);
-    const double ax = fabs(x);
-    double ps, pc, zs = kZivSin, zc = kZivCos;
-    if (ax <= kQuickShort) {
-        ps = fma(fma(x2, kS9h, kS7h), x2, kS5h);
-        pc = fma(fma(x2, kC8h, kC6h), x2, kC4h);
-    } else if (ax > kQuickTight) {
+    double ps, pc;
+    if constexpr (kTier == 0) {
+        ps = fma(fma(x2, L.s9, kS7h), x2, kS5h);
+        pc = fma(fma(x2, L.c8, kC6h), x2, kC4h);
+    } else if constexpr (kTier == 1) {
+        ps = fma(x2, kS13h, kS11h);
+        ps = fma(ps, x2, kS9h);
+        ps = fma(ps, x2, kS7h);
+        ps = fma(ps, x2, kS5h);
+        pc = fma(x2, kC12h, kC10h);
+        pc = fma(pc, x2, kC8h);
+        pc = fma(pc, x2, kC6h);
+        pc = fma(pc, x2, kC4h);
+    } else {
         ps = fma(x2, kS15h, kS13h);
         ps = fma(ps, x2, kS11h);
         ps = fma(ps, x2, kS9h);
@@ -214,18 +238,9 @@ __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
         pc = fma(pc, x2, kC8h);
         pc = fma(pc, x2, kC6h);
         pc = fma(pc, x2, kC4h);
-        zs = kZivWideSin;
-        zc = kZivWideCos;
-    } else {
-        ps = fma(x2, kS13h, kS11h);
-        ps = fma(ps, x2, kS9h);
-        ps = fma(ps, x2, kS7h);
-        ps = fma(ps, x2, kS5h);
-        pc = fma(x2, kC12h, kC10h);
-        pc = fma(pc, x2, kC8h);
-        pc = fma(pc, x2, kC6h);
-        pc = fma(pc, x2, kC4h);
     }
+    const double zs = kTier == 2 ? kZivWideSin : (kTier == 1 ? kZivSin : L.zs);
+    const double zc = kTier == 2 ? kZivWideCos : (kTier == 1 ? kZivCos : L.zc);
     const double t2 = (x3 * x2) * ps;
     const Pair th = fast_two_sum(t1, t2);
     const Pair sh = fast_two_sum(x, th.hi);
@@ -239,6 +254,14 @@ __host__ __device__ inline bool sincos_quick(double x, double& s, double& c) {
     s = sh.hi;
     c = ch.hi;
     return rounds_to_hi(sh.hi, slo, zs) & rounds_to_hi(ch.hi, clo, zc);
+}
+
+// The quick phase for 2^-27 <= |x| <= 2^-2 (the tiers above).
+__host__ __device__ inline bool sincos_quick(double x, double& s, double& c, const QuickLead& L = kQuickLead) {
+    const double ax = fabs(x);
+    if (ax <= kQuickShort) return sincos_quick_tier<0>(x, s, c, L);
+    if (ax <= kQuickTight) return sincos_quick_tier<1>(x, s, c, L);
+    return sincos_quick_tier<2>(x, s, c, L);
 }
 
 // Accurate phase for |x| < 2^20: reduction to y = x - k pi/2 as a pair,
@@ -326,7 +349,11 @@ __host__ __device__ inline bool sincos_accurate(double x, double& s, double& c) 
 // coefficients) does not occupy the caller's registers at every call site.
 __host__ __device__ __attribute__((noinline)) inline Pair sincos_slow(double x) {
     double s, c;
-    if (fabs(x) < 0x1p20) {
+    if (fabs(x) < 0x1p-27) {
+        // sin x rounds to x and cos x to 1
+        s = x;
+        c = 1.0;
+    } else if (fabs(x) < 0x1p20) {
         sincos_accurate(x, s, c);
     } else {
         s = sin(x);
@@ -337,15 +364,12 @@ __host__ __device__ __attribute__((noinline)) inline Pair sincos_slow(double x) 
 
 // sin x and cos x, correctly rounded for |x| < 2^20 (except the undecided
 // accurate-phase cases above, never met in the tests).  Below 2^-27 sin x
-// rounds to x and cos x to 1.
-__host__ __device__ inline void sincos_cr(double x, double* s, double* c) {
+// rounds to x and cos x to 1 (in sincos_slow: kept off the common path, so
+// its results are not set up at every call).
+// L: kQuickLead, or a copy of it held in registers.
+__host__ __device__ inline void sincos_cr(double x, double* s, double* c, const QuickLead& L = kQuickLead) {
     const double ax = fabs(x);
-    if (ax < 0x1p-27) {
-        *s = x;
-        *c = 1.0;
-        return;
-    }
-    if (ax <= kQuickMax && sincos_quick(x, *s, *c)) return;
+    if ((ax >= 0x1p-27) & (ax <= kQuickMax) && sincos_quick(x, *s, *c, L)) return;
     const Pair r = sincos_slow(x);
     *s = r.hi;
     *c = r.lo;

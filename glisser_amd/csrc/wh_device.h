@@ -180,10 +180,13 @@ __host__ __device__ inline bool fp64_core_constants(double mu, double dt) {
 // 2^-54 in magnitude, and the final F' is guarded >= 2^-50.  e sin E0 =
 // r.v / (n a^2) (r.v may be arbitrarily small) and fdot's quotient (sin dE
 // may be) keep the IEEE divide.
+// ql: the correctly rounded sin/cos's short-tier constants (cr::QuickLead),
+// possibly held in registers by the caller.
 template <bool kCrTrig = true, bool kCore = false>
 __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double rz, double vx, double vy,
                                                    double vz, double dt, double mu, double& f, double& g,
-                                                   double& fdot, double& gdot, bool* ok = nullptr) {
+                                                   double& fdot, double& gdot, bool* ok = nullptr,
+                                                   const cr::QuickLead& ql = cr::kQuickLead) {
 #pragma clang fp contract(off)
     const double r2 = rx * rx + ry * ry + rz * rz;
     const double v2 = vx * vx + vy * vy + vz * vz;
@@ -225,11 +228,11 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
     }
     const double adv_dt = kCore ? core_div_pos(dM, n_) : dM / n_;
     double sM, cM;
-    if constexpr (kCrTrig) cr::sincos_cr(dM, &sM, &cM);
+    if constexpr (kCrTrig) cr::sincos_cr(dM, &sM, &cM, ql);
     else sincos(dM, &sM, &cM);
     double dE = dM - esinEo + esinEo * cM + ecosEo * sM;
     double sindE, cosdE;
-    if constexpr (kCrTrig) cr::sincos_cr(dE, &sindE, &cosdE);
+    if constexpr (kCrTrig) cr::sincos_cr(dE, &sindE, &cosdE, ql);
     else sincos(dE, &sindE, &cosdE);
     double delta = 0.0;
 #pragma unroll
@@ -243,7 +246,7 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
             delta = -fk / fpk;
         }
         dE += delta;
-        if constexpr (kCrTrig) cr::sincos_cr(dE, &sindE, &cosdE);
+        if constexpr (kCrTrig) cr::sincos_cr(dE, &sindE, &cosdE, ql);
         else sincos(dE, &sindE, &cosdE);
     }
     const double fp = 1.0 - ecosEo * cosdE + esinEo * sindE;
@@ -266,13 +269,15 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
 // fp64_core_constants of the launch): the guarded divide and square-root
 // cores, with the lanes outside their guards recomputed with the IEEE
 // operators (the same bits either way).
-__device__ __forceinline__ void drift_fp64(Particle& p, double dt, double mu, bool core) {
+__device__ __forceinline__ void drift_fp64(Particle& p, double dt, double mu, bool core,
+                                           const cr::QuickLead& ql = cr::kQuickLead) {
 #pragma clang fp contract(off)
     double f, g, fdot, gdot;
     bool ok = core;
     uint32_t fl = 0;
-    if (core) fl = kepler_fg_fp64<true, true>(p.rx, p.ry, p.rz, p.vx, p.vy, p.vz, dt, mu, f, g, fdot, gdot, &ok);
-    if (!ok) fl = kepler_fg_fp64(p.rx, p.ry, p.rz, p.vx, p.vy, p.vz, dt, mu, f, g, fdot, gdot);
+    if (core)
+        fl = kepler_fg_fp64<true, true>(p.rx, p.ry, p.rz, p.vx, p.vy, p.vz, dt, mu, f, g, fdot, gdot, &ok, ql);
+    if (!ok) fl = kepler_fg_fp64(p.rx, p.ry, p.rz, p.vx, p.vy, p.vz, dt, mu, f, g, fdot, gdot, nullptr, ql);
     p.flags = fl;
     if (p.flags == kFlagUnbound) return;
     const double rx0 = p.rx, ry0 = p.ry, rz0 = p.rz;

@@ -101,8 +101,24 @@ __device__ __forceinline__ void sincos_taylor(double x, double& s, double& cm1,
 // register at every step.
 struct TaylorLead {
     double s7, c6;
+    cr::QuickLead q;  // fp64 form: the correctly rounded sin/cos's short-tier constants
 };
-__device__ __forceinline__ TaylorLead taylor_lead() { return {kTaylor[0][2], kTaylor[1][2]}; }
+__device__ __forceinline__ TaylorLead taylor_lead() { return {kTaylor[0][2], kTaylor[1][2], cr::kQuickLead}; }
+
+// Holds a kernel's lead constants in vector registers: the fast forms'
+// start-polynomial coefficients, or the fp64 form's sin/cos constants.
+template <int FORM>
+__device__ __forceinline__ void pin_lead(TaylorLead& lead) {
+    if constexpr (FORM != GLX_FORM_FP64) {
+        pin(lead.s7);
+        pin(lead.c6);
+    } else {
+        pin(lead.q.s9);
+        pin(lead.q.c8);
+        pin(lead.q.zs);
+        pin(lead.q.zc);
+    }
+}
 
 // sincos_taylor<3> with the highest coefficients from `lead`, returning
 // sin x - x (smx, whose rounding then stays relative to x^3 / 6) instead of
@@ -444,7 +460,7 @@ __device__ __forceinline__ void particle_step(Particle& p, uint32_t& dti, const 
     if (p.flags != 0) return;
     if constexpr (FORM == GLX_FORM_FP64) {
         kick(p, A.half_dt);
-        drift_fp64(p, A.dt, A.mu, A.fp64_core != 0);
+        drift_fp64(p, A.dt, A.mu, A.fp64_core != 0, lead.q);
         accel_fp64_step<NB, NMAX>(p, rowp, A.pc, A.outer2, nb, A.fp64_core != 0);
         kick(p, A.half_dt);
         if (p.flags != 0) dti = death;

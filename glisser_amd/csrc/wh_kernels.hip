@@ -74,10 +74,7 @@ __global__ __launch_bounds__(256) void wh_block_kernel(BlockArgs A) {
     // FAST / MIXED: the start polynomials' highest coefficients in vector
     // registers
     TaylorLead lead = taylor_lead();
-    if constexpr (FORM != GLX_FORM_FP64) {
-        pin(lead.s7);
-        pin(lead.c6);
-    }
+    pin_lead<FORM>(lead);
     bool pending = false;  // FAST / MIXED: closing half kick still due
     for (uint32_t cs = 0; cs < A.n_steps; cs += A.chunk) {
         const uint32_t cn = min(A.chunk, A.n_steps - cs);
