@@ -58,7 +58,7 @@ HEADLINE_FORM = "mixed"
 # summed over the block kernel and divided by its particle-steps); see
 # profiles/wh_block_200k_counters.md.  Used with the live rate to report
 # the counter-based share of FP64 vector peak.
-COUNTER_FLOPS_PER_PSTEP = {"fp64": 948.25, "fast": 210.34, "mixed": 206.34}
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 900.55, "fast": 210.34, "mixed": 206.34}
 
 # VALU-busy SIMD cycles per particle-step (rocprofv3 SQ_ACTIVE_INST_VALU, in
 # quad-cycles, x 4 / particle-steps, one 200k-particle block in one
@@ -67,8 +67,8 @@ COUNTER_FLOPS_PER_PSTEP = {"fp64": 948.25, "fast": 210.34, "mixed": 206.34}
 # profiles/wh_block_200k_counters.md.  Used with the live rate to report the
 # share of the SIMDs' issue cycles the kernel keeps busy and the roofline
 # fraction at the measured clock.
-VALU_CYCLES_PER_PSTEP = {"fp64": 66.23, "fast": 10.545, "mixed": 10.545}
-CLOCK_GHZ_MEASURED = {"fp64": 2.274, "fast": 2.207, "mixed": 2.245}
+VALU_CYCLES_PER_PSTEP = {"fp64": 55.62, "fast": 10.545, "mixed": 10.545}
+CLOCK_GHZ_MEASURED = {"fp64": 2.177, "fast": 2.227, "mixed": 2.183}
 N_SIMDS = 1024  # 256 CUs x 4 SIMDs
 PEAK_CLOCK_GHZ = 2.4
 
@@ -77,7 +77,7 @@ PEAK_CLOCK_GHZ = 2.4
 # slice's state once), from rocprofv3 FETCH_SIZE (x 2.0, the factor a copy
 # kernel of known size needs) and WRITE_SIZE (x 1.0); see
 # profiles/wh_block_200k_counters.md.
-TRAFFIC_BYTES_PER_BLOCK = {"mixed": 133.3e6}
+TRAFFIC_BYTES_PER_BLOCK = {"mixed": 133.4e6}
 
 TRACK_RECORD_BYTES = 28  # u32 id + 6 float32 per particle in a track snapshot
 
@@ -135,7 +135,7 @@ def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int) -> int:
 # kernel is the profiled one.
 KERNEL_SOURCES = ["glisser_amd/csrc/wh_kernels.hip", "glisser_amd/csrc/wh_fast.h", "glisser_amd/csrc/wh_device.h",
                   "glisser_amd/csrc/sincos_cr.h", "glisser_amd/csrc/launch.h"]
-PROFILED_KERNEL_SHA256 = "c03cb9067d8ff0a8083b3140e354da65c06612876de26a4275f57169f7b56136"
+PROFILED_KERNEL_SHA256 = "56e2fadb30014e240dbe8e9dee627d4c7b4e68267f82b7c5b32dfc227fc69ca6"
 
 
 def kernel_fingerprint() -> str:
