@@ -200,6 +200,19 @@ struct QuickLead {
 };
 constexpr QuickLead kQuickLead = {kS9h, kC8h, kZivSin, kZivCos};
 
+// fma(a, b, c), on the device as one three-operand v_fma_f64 with every
+// operand in a vector register: where the addend is a constant the compiler
+// otherwise copies it into the destination for the two-operand form.
+__host__ __device__ inline double fma3(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    __asm__("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return fma(a, b, c);
+#endif
+}
+
 // One tier of the quick phase (0: |x| <= 2^-5, 1: |x| <= 2^-4, 2: |x| <=
 // 2^-2), evaluated straight through: each tier is its own code path, so its
 // polynomial coefficients and rounding-test factors stay constants (tier 0:
@@ -216,8 +229,8 @@ __host__ __device__ inline bool sincos_quick_tier(double x, double& s, double& c
     const double t1l = fma(kS3h, x3, -t1) + fma(kS3h, x3l, kS3l * x3);
     double ps, pc;
     if constexpr (kTier == 0) {
-        ps = fma(fma(x2, L.s9, kS7h), x2, kS5h);
-        pc = fma(fma(x2, L.c8, kC6h), x2, kC4h);
+        ps = fma3(fma3(x2, L.s9, kS7h), x2, kS5h);
+        pc = fma3(fma3(x2, L.c8, kC6h), x2, kC4h);
     } else if constexpr (kTier == 1) {
         ps = fma(x2, kS13h, kS11h);
         ps = fma(ps, x2, kS9h);
