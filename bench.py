@@ -63,12 +63,13 @@ COUNTER_FLOPS_PER_PSTEP = {"fp64": 900.55, "fast": 210.34, "mixed": 206.34}
 # VALU-busy SIMD cycles per particle-step (rocprofv3 SQ_ACTIVE_INST_VALU, in
 # quad-cycles, x 4 / particle-steps, one 200k-particle block in one
 # dispatch) and the shader clock during the kernel (GRBM_GUI_ACTIVE summed
-# over the 8 XCDs / 8 / dispatch duration); see
+# over the 8 XCDs / 8 / dispatch duration: the highest seen on any box, so
+# the figures derived from it err low); see
 # profiles/wh_block_200k_counters.md.  Used with the live rate to report the
 # share of the SIMDs' issue cycles the kernel keeps busy and the roofline
 # fraction at the measured clock.
-VALU_CYCLES_PER_PSTEP = {"fp64": 55.62, "fast": 10.545, "mixed": 10.545}
-CLOCK_GHZ_MEASURED = {"fp64": 2.177, "fast": 2.227, "mixed": 2.183}
+VALU_CYCLES_PER_PSTEP = {"fp64": 53.87, "fast": 10.545, "mixed": 10.545}
+CLOCK_GHZ_MEASURED = {"fp64": 2.279, "fast": 2.229, "mixed": 2.257}
 N_SIMDS = 1024  # 256 CUs x 4 SIMDs
 PEAK_CLOCK_GHZ = 2.4
 
@@ -135,7 +136,7 @@ def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int) -> int:
 # kernel is the profiled one.
 KERNEL_SOURCES = ["glisser_amd/csrc/wh_kernels.hip", "glisser_amd/csrc/wh_fast.h", "glisser_amd/csrc/wh_device.h",
                   "glisser_amd/csrc/sincos_cr.h", "glisser_amd/csrc/launch.h"]
-PROFILED_KERNEL_SHA256 = "56e2fadb30014e240dbe8e9dee627d4c7b4e68267f82b7c5b32dfc227fc69ca6"
+PROFILED_KERNEL_SHA256 = "4a8dcc3f2b3be98c877a485fe608f9394d3321d1a69d73439c49cd81a1a7906c"
 
 
 def kernel_fingerprint() -> str:
