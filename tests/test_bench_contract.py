@@ -26,6 +26,18 @@ def test_flops_per_particle_step(bench):
     assert bench.flops_per_particle_step(1) == 151 + 17 + 3 + 14
 
 
+def test_profiled_counters_match_the_kernel_sources(bench):
+    """The counter-based fields of the bench line (counter FP64 share, VALU
+    busy, HBM traffic, roofline fraction at the measured clock) come from a
+    profile of the kernel sources whose SHA-256 bench.py records; the
+    sources must still be those, or the profile is stale."""
+    assert bench.PROFILED_KERNEL_SHA256 == bench.kernel_fingerprint(), (
+        "kernel sources changed since profiles/wh_block_200k_counters.md: re-profile and update bench.py")
+    for table in (bench.COUNTER_FLOPS_PER_PSTEP, bench.VALU_CYCLES_PER_PSTEP, bench.CLOCK_GHZ_MEASURED):
+        assert set(table) == {"fp64", "fast", "mixed"}
+    assert all(2.0 < c <= bench.PEAK_CLOCK_GHZ for c in bench.CLOCK_GHZ_MEASURED.values())
+
+
 def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
