@@ -58,18 +58,19 @@ HEADLINE_FORM = "mixed"
 # summed over the block kernel and divided by its particle-steps); see
 # profiles/wh_block_200k_counters.md.  Used with the live rate to report
 # the counter-based share of FP64 vector peak.
-COUNTER_FLOPS_PER_PSTEP = {"fp64": 900.55, "fast": 210.34, "mixed": 206.34}
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 881.36, "fast": 210.34, "mixed": 206.34}
 
 # VALU-busy SIMD cycles per particle-step (rocprofv3 SQ_ACTIVE_INST_VALU, in
 # quad-cycles, x 4 / particle-steps, one 200k-particle block in one
-# dispatch) and the shader clock during the kernel (GRBM_GUI_ACTIVE summed
-# over the 8 XCDs / 8 / dispatch duration: the highest seen on any box, so
-# the figures derived from it err low); see
+# dispatch) and the shader clock during the benchmark's launches (the
+# highest seen on any box, by GRBM_GUI_ACTIVE in the counter runs or by
+# s_memtime / s_memrealtime read at each workgroup's start and end in an
+# instrumented build, so the figures derived from it err low); see
 # profiles/wh_block_200k_counters.md.  Used with the live rate to report the
 # share of the SIMDs' issue cycles the kernel keeps busy and the roofline
 # fraction at the measured clock.
-VALU_CYCLES_PER_PSTEP = {"fp64": 53.87, "fast": 10.545, "mixed": 10.545}
-CLOCK_GHZ_MEASURED = {"fp64": 2.279, "fast": 2.229, "mixed": 2.257}
+VALU_CYCLES_PER_PSTEP = {"fp64": 52.90, "fast": 10.545, "mixed": 10.545}
+CLOCK_GHZ_MEASURED = {"fp64": 2.370, "fast": 2.280, "mixed": 2.289}
 N_SIMDS = 1024  # 256 CUs x 4 SIMDs
 PEAK_CLOCK_GHZ = 2.4
 
@@ -136,7 +137,7 @@ def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int) -> int:
 # kernel is the profiled one.
 KERNEL_SOURCES = ["glisser_amd/csrc/wh_kernels.hip", "glisser_amd/csrc/wh_fast.h", "glisser_amd/csrc/wh_device.h",
                   "glisser_amd/csrc/sincos_cr.h", "glisser_amd/csrc/launch.h"]
-PROFILED_KERNEL_SHA256 = "4a8dcc3f2b3be98c877a485fe608f9394d3321d1a69d73439c49cd81a1a7906c"
+PROFILED_KERNEL_SHA256 = "1bb318ebe9e4f03321579b847d922f41069abab75d37d72ca49f76eda0564fff"
 
 
 def kernel_fingerprint() -> str:

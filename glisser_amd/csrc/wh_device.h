@@ -245,9 +245,16 @@ __device__ __forceinline__ uint32_t kepler_fg_fp64(double rx, double ry, double 
         } else {
             delta = -fk / fpk;
         }
+        const double prev = dE;
         dE += delta;
-        if constexpr (kCrTrig) cr::sincos_cr(dE, &sindE, &cosdE, ql);
-        else sincos(dE, &sindE, &cosdE);
+        // sin and cos depend on the argument's bits only: in the last two
+        // iterations the iterate has mostly stopped moving, and a lane whose
+        // dE kept its bits keeps its sin and cos (a wave skips the
+        // evaluation when all of its lanes did)
+        if (k < kKepIts - 2 || __double_as_longlong(dE) != __double_as_longlong(prev)) {
+            if constexpr (kCrTrig) cr::sincos_cr(dE, &sindE, &cosdE, ql);
+            else sincos(dE, &sindE, &cosdE);
+        }
     }
     const double fp = 1.0 - ecosEo * cosdE + esinEo * sindE;
     if constexpr (kCore) {
