@@ -56,30 +56,29 @@ HEADLINE_FORM = "mixed"
 # FP64 FLOPs per particle-step counted by rocprofv3 on this build
 # (64 x (ADD_F64 + MUL_F64 + TRANS_F64 + 2 FMA_F64) per wave instruction,
 # summed over the block kernel and divided by its particle-steps); see
-# profiles/wh_block_200k_counters.md.  Used with the live rate to report
+# profiles/wh_queue_200k_counters.md.  Used with the live rate to report
 # the counter-based share of FP64 vector peak.
-COUNTER_FLOPS_PER_PSTEP = {"fp64": 881.36, "fast": 210.34, "mixed": 206.34}
+COUNTER_FLOPS_PER_PSTEP = {"fp64": 880.36, "fast": 210.39, "mixed": 206.39}
 
 # VALU-busy SIMD cycles per particle-step (rocprofv3 SQ_ACTIVE_INST_VALU, in
-# quad-cycles, x 4 / particle-steps, one 200k-particle block in one
-# dispatch) and the shader clock during the benchmark's launches (the
-# highest seen on any box, by GRBM_GUI_ACTIVE in the counter runs or by
-# s_memtime / s_memrealtime read at each workgroup's start and end in an
-# instrumented build, so the figures derived from it err low); see
-# profiles/wh_block_200k_counters.md.  Used with the live rate to report the
-# share of the SIMDs' issue cycles the kernel keeps busy and the roofline
-# fraction at the measured clock.
-VALU_CYCLES_PER_PSTEP = {"fp64": 52.90, "fast": 10.545, "mixed": 10.545}
+# quad-cycles, x 4 / particle-steps, the benchmark's queued launch) and the
+# shader clock during the benchmark's launches (the highest seen on any box,
+# by GRBM_GUI_ACTIVE in the counter runs or by s_memtime / s_memrealtime
+# read at each workgroup's start and end in an instrumented build, so the
+# figures derived from it err low); see profiles/wh_queue_200k_counters.md.
+# Used with the live rate to report the share of the SIMDs' issue cycles the
+# kernel keeps busy and the roofline fraction at the measured clock.
+VALU_CYCLES_PER_PSTEP = {"fp64": 51.87, "fast": 10.588, "mixed": 10.588}
 CLOCK_GHZ_MEASURED = {"fp64": 2.370, "fast": 2.280, "mixed": 2.289}
 N_SIMDS = 1024  # 256 CUs x 4 SIMDs
 PEAK_CLOCK_GHZ = 2.4
 
-# HBM bytes per time-block of the mixed form at this workload and launch
-# shape (4 slices x 4 launches per block: each launch reads and writes its
-# slice's state once), from rocprofv3 FETCH_SIZE (x 2.0, the factor a copy
-# kernel of known size needs) and WRITE_SIZE (x 1.0); see
-# profiles/wh_block_200k_counters.md.
-TRAFFIC_BYTES_PER_BLOCK = {"mixed": 133.4e6}
+# HBM bytes per time-block of the mixed form at this workload, by launch
+# shape (queued: every 256-step task reads and writes its tile's state, 8
+# times per block; sliced: 4 slices x 4 launches per block, 4 times), from
+# rocprofv3 FETCH_SIZE (x 2.0, the factor a copy kernel of known size needs)
+# and WRITE_SIZE (x 1.0); see profiles/wh_queue_200k_counters.md.
+TRAFFIC_BYTES_PER_BLOCK = {("mixed", "queued"): 247.2e6, ("mixed", "sliced"): 133.4e6}
 
 TRACK_RECORD_BYTES = 28  # u32 id + 6 float32 per particle in a track snapshot
 
@@ -137,7 +136,7 @@ def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int) -> int:
 # kernel is the profiled one.
 KERNEL_SOURCES = ["glisser_amd/csrc/wh_kernels.hip", "glisser_amd/csrc/wh_fast.h", "glisser_amd/csrc/wh_device.h",
                   "glisser_amd/csrc/sincos_cr.h", "glisser_amd/csrc/launch.h"]
-PROFILED_KERNEL_SHA256 = "1bb318ebe9e4f03321579b847d922f41069abab75d37d72ca49f76eda0564fff"
+PROFILED_KERNEL_SHA256 = "66cec6244dc5a6ceafed29669800dcf438d9ce82a978a09ec9b23eeb98bd97eb"
 
 
 def kernel_fingerprint() -> str:
@@ -189,7 +188,7 @@ def prepare_gpu(inputs, steps: int, warmup: int):
 
 
 def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_simd: int, parts: int = 1,
-            ctx=None):
+            ctx=None, launch: str = "sliced", segment_steps: int = 0):
     """`warmup` untimed blocks, then the timed region: the planet rows of
     `steps` blocks interpolated on the device, one launch of the `steps`
     blocks, the encounter-kill transform, the step count and the stable
@@ -202,7 +201,10 @@ def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_s
         if warmup:
             for k in range(warmup):
                 ctx.interp_block(k, k * TBSIZE * DT, DT, TBSIZE, rh)
-            ctx.run_blocks(warmup, 0, DT, OUTER, form, shards, waves_per_simd, parts)
+            if launch == "queued":
+                ctx.run_blocks_queued(warmup, 0, DT, OUTER, form, segment_steps)
+            else:
+                ctx.run_blocks(warmup, 0, DT, OUTER, form, shards, waves_per_simd, parts)
             ctx.kill_encounters()
             ctx.partition()
         ctx.sync()
@@ -210,7 +212,10 @@ def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_s
         t0 = time.perf_counter()
         for k in range(warmup, warmup + steps):
             ctx.interp_block(k, k * TBSIZE * DT, DT, TBSIZE, rh)
-        ctx.run_blocks(steps, warmup, DT, OUTER, form, shards, waves_per_simd, parts)
+        if launch == "queued":
+            ctx.run_blocks_queued(steps, warmup, DT, OUTER, form, segment_steps)
+        else:
+            ctx.run_blocks(steps, warmup, DT, OUTER, form, shards, waves_per_simd, parts)
         ctx.kill_encounters()
         psteps = ctx.sum_steps(n_alive)
         ctx.partition()
@@ -661,6 +666,11 @@ def main(argv=None):
                     help="resident-wave cap (0: none; default: 4 for fp64, none for fast and mixed)")
     ap.add_argument("--launches-per-block", type=int, default=4,
                     help="launches each 2048-step block is split into (same results, shorter chain end)")
+    ap.add_argument("--launch", default="queued", choices=["queued", "sliced"],
+                    help="queued: one persistent launch taking (tile, segment) tasks from a device queue; "
+                         "sliced: --shards streams of --launches-per-block launches per block")
+    ap.add_argument("--segment-steps", type=int, default=0,
+                    help="steps per task of the queued launch (multiple of 64; 0: 256)")
     ap.add_argument("--cpu-sample", type=int, default=16000,
                     help="particles in the single-thread CPU-baseline sample (~4 s per run, 3 runs)")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -701,7 +711,8 @@ def main(argv=None):
     ctxs = {form: prepare_gpu(inputs, args.steps, args.warmup) for form in forms}
     for form in forms:
         r = run_gpu(inputs, form, args.steps, args.warmup, args.shards, waves_per_simd(args, form),
-                    args.launches_per_block, ctx=ctxs.pop(form))
+                    args.launches_per_block, ctx=ctxs.pop(form), launch=args.launch,
+                    segment_steps=args.segment_steps)
         finals[form] = r.pop("final")
         block_s = r["kernel_ms"] / 1e3 / args.steps
         per_block = r["particle_steps"] / args.steps
@@ -747,9 +758,10 @@ def main(argv=None):
             "dt_days": DT,
             "form": head,
             "blocks_per_launch": args.steps,
-            "shards": args.shards,
-            "waves_per_simd": waves_per_simd(args, head),
-            "launches_per_block": args.launches_per_block,
+            "launch": args.launch,
+            **({"segment_steps": args.segment_steps or 256} if args.launch == "queued" else
+               {"shards": args.shards, "waves_per_simd": waves_per_simd(args, head),
+                "launches_per_block": args.launches_per_block}),
             "planet_log": f"device cubic-Hermite interpolation of a {HIST_SPACING:g}-d history",
             "seed": args.seed,
             "parallelism": "single-gpu",
@@ -763,15 +775,16 @@ def main(argv=None):
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": h["alg_tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": (TRAFFIC_BYTES_PER_BLOCK[head] * args.steps
-                        if head in TRAFFIC_BYTES_PER_BLOCK and args.particles == N_PARTICLES and counters_current
+            "traffic": (TRAFFIC_BYTES_PER_BLOCK[head, args.launch] * args.steps
+                        if (head, args.launch) in TRAFFIC_BYTES_PER_BLOCK and args.particles == N_PARTICLES
+                        and counters_current and args.segment_steps in (0, 256)
                         else None),
             "traffic_note": "HBM bytes per launch of `steps` blocks (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE)",
             "counter_fp64_frac": h.get("counter_fp64_frac"),
             "clock_ghz_measured": CLOCK_GHZ_MEASURED[head] if counters_current else None,
             "frac_at_measured_clock": h.get("frac_at_measured_clock"),
             "valu_busy": h.get("valu_busy"),
-            "counters_note": ("from the counters of profiles/wh_block_200k_counters.md (same kernel sources)"
+            "counters_note": ("from the counters of profiles/wh_queue_200k_counters.md (same kernel sources)"
                               if counters_current else
                               "kernel sources differ from the profiled build: counter-based fields omitted"),
             "flops_per_particle_step": f_alg,

@@ -67,6 +67,11 @@ struct PlanetBuf {
 
 constexpr uint32_t kQueryMax = 1024;  // times per glx_interp_at batch
 
+constexpr uint32_t kQueueSegment = 256;             // default steps per task of a queued launch
+constexpr uint32_t kQueueSegmentMax = 4096;         // longest task: a slot wait lasts at most one task
+constexpr uint64_t kQueueWaitTicks = 3000000000ull; // 30 s of the 100 MHz counter: bound of a FIFO slot wait
+constexpr uint32_t kQueueErrWord = 8;               // pin_small slot of a queued launch's error word
+
 // Sampled planet history resident on the device (glx_upload_history).
 struct History {
     double* t = nullptr;
@@ -123,6 +128,16 @@ struct glx_ctx {
     hipStream_t dth_s = nullptr;      // device -> host copies of staged records
     hipEvent_t enc_packed = nullptr;  // records packed on the main stream
     hipEvent_t enc_copied = nullptr;  // records in the pinned staging buffer
+    // Queued launches (glx_run_blocks_queued), allocated on first use:
+    // segment and block tables, counters + error word, the task FIFO, and
+    // the tables' pinned staging.
+    glx::QueueSeg* q_segs = nullptr;
+    glx::QueueBlock* q_blocks = nullptr;
+    uint32_t* q_ctl = nullptr;
+    uint32_t* q_ring = nullptr;
+    unsigned char* q_pin = nullptr;
+    hipEvent_t q_staged = nullptr;  // the staging's last copy has completed
+    bool q_pending = false;         // a queued launch's error word is bound for pin_small[kQueueErrWord]
     std::vector<void*> allocs;
 };
 
@@ -175,6 +190,8 @@ void release(glx_ctx* c) {
         if (b.consumed) (void)hipEventDestroy(b.consumed);
     }
     if (c->pin_small) (void)hipHostFree(c->pin_small);
+    if (c->q_pin) (void)hipHostFree(c->q_pin);
+    if (c->q_staged) (void)hipEventDestroy(c->q_staged);
     for (hipEvent_t e : {c->k0, c->k1, c->q0, c->q1})
         if (e) (void)hipEventDestroy(e);
     for (uint32_t k = 0; k < kMaxShards; k++) {
@@ -591,6 +608,122 @@ int glx_run_blocks(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, doub
     return GLX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Segment-table capacity of a queued launch: every planet buffer in one
+// launch, cut into segments of at least kRadiusPeriod steps.
+size_t queue_seg_cap(const glx_ctx* c) {
+    return c->pb.size() * (size_t)((c->tb_max + glx::kRadiusPeriod - 1) / glx::kRadiusPeriod);
+}
+
+int ensure_queue(glx_ctx* c) {
+    if (c->q_segs) return GLX_OK;
+    const size_t segs = queue_seg_cap(c), blocks = c->pb.size();
+    const size_t tiles = (c->cap + 255) / 256 + 1;
+    int rc;
+    if ((rc = dalloc(c, &c->q_segs, segs)) || (rc = dalloc(c, &c->q_blocks, blocks)) ||
+        (rc = dalloc(c, &c->q_ctl, 3)) || (rc = dalloc(c, &c->q_ring, tiles * segs)))
+        return rc;
+    const size_t pin = segs * sizeof(glx::QueueSeg) + blocks * sizeof(glx::QueueBlock);
+    if (hipHostMalloc((void**)&c->q_pin, pin, hipHostMallocDefault) != hipSuccess) {
+        c->q_pin = nullptr;
+        return fail(GLX_ENOMEM, "hipHostMalloc(queue tables) failed");
+    }
+    if (hipEventCreateWithFlags(&c->q_staged, hipEventDisableTiming) != hipSuccess) {
+        c->q_staged = nullptr;
+        return fail(GLX_EHIP, "hipEventCreate failed");
+    }
+    return GLX_OK;
+}
+
+// After the main stream has been synchronized: the error word of the last
+// queued launch, if one is outstanding.
+int check_queue(glx_ctx* c) {
+    if (!c->q_pending) return GLX_OK;
+    c->q_pending = false;
+    if (c->pin_small[kQueueErrWord] != 0)
+        return fail(GLX_EHIP, "queued launch: a task-queue slot was not filled within the wait bound; "
+                              "the particles' state is incomplete");
+    return GLX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glx_run_blocks_queued(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
+                          uint32_t segment_steps) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_form(form)) return rc;
+    if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
+    const int nbuf = (int)c->pb.size();
+    if (n_blocks < 1 || (int)n_blocks > nbuf)
+        return fail(GLX_EINVAL, "n_blocks %u outside [1, %d planet buffers]", n_blocks, nbuf);
+    if (segment_steps == 0) segment_steps = kQueueSegment;
+    if (segment_steps % glx::kRadiusPeriod != 0 || segment_steps > kQueueSegmentMax)
+        return fail(GLX_EINVAL, "segment_steps %u is not a multiple of %u up to %u", segment_steps,
+                    glx::kRadiusPeriod, kQueueSegmentMax);
+    if (first_buf < 0) return fail(GLX_EINVAL, "first_buf %d is negative", first_buf);
+    const PlanetBuf& b0 = c->pb[first_buf % nbuf];
+    for (uint32_t k = 0; k < n_blocks; k++) {
+        const int buf = (first_buf + (int)k) % nbuf;
+        if (int rc = check_buf(c, buf)) return rc;
+        if (c->pb[buf].n_bodies != b0.n_bodies)
+            return fail(GLX_EINVAL, "planet buffers of one launch must hold the same bodies");
+        if (c->pb[buf].mu != b0.mu)
+            return fail(GLX_EINVAL, "planet buffers of one launch must hold the same central mass");
+    }
+    GLX_HIP(hipSetDevice(c->device));
+    if (int rc = ensure_queue(c)) return rc;
+    // The staging may still feed the previous launch's table copy.
+    GLX_HIP(hipEventSynchronize(c->q_staged));
+    glx::QueueSeg* segs = reinterpret_cast<glx::QueueSeg*>(c->q_pin);
+    glx::QueueBlock* blocks = reinterpret_cast<glx::QueueBlock*>(c->q_pin + queue_seg_cap(c) * sizeof(glx::QueueSeg));
+    uint32_t n_segs = 0, offset = 0;
+    for (uint32_t k = 0; k < n_blocks; k++) {
+        const PlanetBuf& b = c->pb[(first_buf + (int)k) % nbuf];
+        blocks[k] = glx::QueueBlock{};
+        blocks[k].log = b.dev_log;
+        blocks[k].step_offset = offset;
+        blocks[k].sun_vote = std::sqrt(b.pc.rh2[0]) * (1.0 + glx::kVoteMargin);
+        blocks[k].pc = b.pc;
+        for (uint32_t r0 = 0; r0 < b.n_steps; r0 += segment_steps)
+            segs[n_segs++] = glx::QueueSeg{k, r0, std::min(segment_steps, b.n_steps - r0), 0};
+        offset += b.n_steps;
+    }
+    glx::QueueArgs Q{};
+    Q.base = block_args(c, b0, dt, outer_radius, kLdsBudget);
+    Q.base.chunk = std::min<uint32_t>(Q.base.chunk, segment_steps);
+    Q.segs = c->q_segs;
+    Q.blocks = c->q_blocks;
+    Q.ctl = c->q_ctl;
+    Q.ring = c->q_ring;
+    Q.n_segs = n_segs;
+    Q.n_tiles = (uint32_t)((c->n_alive + 255) / 256);
+    Q.wait_ticks = kQueueWaitTicks;
+    const size_t seg_bytes = (size_t)n_segs * sizeof(glx::QueueSeg);
+    const size_t blk_bytes = (size_t)n_blocks * sizeof(glx::QueueBlock);
+    GLX_HIP(hipMemcpyAsync(c->q_segs, segs, seg_bytes, hipMemcpyHostToDevice, c->main_s));
+    GLX_HIP(hipMemcpyAsync(c->q_blocks, blocks, blk_bytes, hipMemcpyHostToDevice, c->main_s));
+    GLX_HIP(hipEventRecord(c->q_staged, c->main_s));
+    for (uint32_t k = 0; k < n_blocks; k++)
+        GLX_HIP(hipStreamWaitEvent(c->main_s, c->pb[(first_buf + (int)k) % nbuf].uploaded, 0));
+    GLX_HIP(hipEventRecord(c->k0, c->main_s));
+    GLX_HIP(glx::launch_queue(Q, form, 0, c->main_s));
+    GLX_HIP(hipEventRecord(c->k1, c->main_s));
+    c->pin_small[kQueueErrWord] = 0;
+    if (Q.n_tiles > 0) {
+        GLX_HIP(hipMemcpyAsync(c->pin_small + kQueueErrWord, c->q_ctl + 2, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               c->main_s));
+        c->q_pending = true;
+    }
+    for (uint32_t k = 0; k < n_blocks; k++) GLX_HIP(hipEventRecord(c->pb[(first_buf + (int)k) % nbuf].consumed, c->main_s));
+    c->kernel_timed = true;
+    return GLX_OK;
+}
+
 int glx_upload_history(glx_ctx* c, uint32_t n_bodies, const double* m, const uint32_t* id, uint64_t n_samples,
                        const double* t, const double* r, const double* v) {
     if (!c) return fail(GLX_EINVAL, "null context");
@@ -821,6 +954,7 @@ int glx_partition(glx_ctx* c, int mode, uint64_t* n_unflagged, uint64_t* n_encou
     }
     GLX_HIP(hipEventRecord(c->q1, c->main_s));
     GLX_HIP(hipStreamSynchronize(c->main_s));
+    if (int rc = check_queue(c)) return rc;
     if (n > 0) {
         n0 = c->pin_small[0];
         n1 = mode == GLX_PART_THREE_WAY ? c->pin_small[1] : 0;
@@ -857,7 +991,7 @@ int glx_sync(glx_ctx* c) {
     GLX_HIP(hipSetDevice(c->device));
     GLX_HIP(hipStreamSynchronize(c->htd_s));
     GLX_HIP(hipStreamSynchronize(c->main_s));
-    return GLX_OK;
+    return check_queue(c);
 }
 
 int glx_last_times(glx_ctx* c, float* kernel_ms, float* partition_ms) {

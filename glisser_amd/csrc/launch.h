@@ -29,6 +29,10 @@ struct RollbackSet {
 // chunk); a larger request caps the workgroups resident per CU.
 hipError_t launch_block(const BlockArgs& A, int form, size_t lds_bytes, hipStream_t stream);
 hipError_t launch_init_accel(const BlockArgs& A, hipStream_t stream);
+// Resets Q's FIFO and counters, then runs every task of Q in one persistent
+// grid (as many workgroups as fit on the device, at most one per tile).
+// lds_bytes as launch_block.
+hipError_t launch_queue(const QueueArgs& Q, int form, size_t lds_bytes, hipStream_t stream);
 
 // Stable partition scratch sizes for n elements.
 constexpr uint32_t kPartTile = 1024;  // elements per workgroup (256 lanes x 4)

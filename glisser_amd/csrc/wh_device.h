@@ -84,6 +84,37 @@ struct BlockArgs {
     PlanetConsts pc;
 };
 
+// Queued multi-block launch (one persistent grid for several consecutive
+// blocks).  A task is one 256-particle tile through one segment of steps.
+// Tasks wait in a FIFO (`ring`) that holds only runnable tasks: it starts
+// with every tile's first segment, and the workgroup that finishes a tile's
+// segment k appends segment k + 1.  Entries are task + 1 (task = segment x
+// n_tiles + tile); 0 marks a slot not written yet.
+struct QueueSeg {
+    uint32_t block;    // index into QueueArgs::blocks
+    uint32_t r0;       // first step of the segment within its block (a multiple of kRadiusPeriod)
+    uint32_t n_steps;
+    uint32_t pad;
+};
+// What differs between the blocks of one queued launch.
+struct QueueBlock {
+    const double* log;     // the block's planet rows
+    uint32_t step_offset;  // steps of the launch before this block
+    uint32_t pad;
+    double sun_vote;
+    PlanetConsts pc;
+};
+struct QueueArgs {
+    BlockArgs base;          // particle arrays and the constants the blocks share
+    const QueueSeg* segs;
+    const QueueBlock* blocks;
+    uint32_t* ctl;           // [0] next slot to take, [1] next slot to fill, [2] nonzero once a wait gave up
+    uint32_t* ring;          // n_segs x n_tiles slots
+    uint32_t n_segs;
+    uint32_t n_tiles;
+    uint64_t wait_ticks;     // bound of a slot wait, in ticks of the 100 MHz real-time counter
+};
+
 struct Particle {
     double rx, ry, rz, vx, vy, vz, ax, ay, az;
     double rl, ir;  // |r| and 1/|r| carried between steps (FAST / MIXED forms only)

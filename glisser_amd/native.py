@@ -150,6 +150,8 @@ SIGNATURES = {
     "glx_run_block": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
     "glx_run_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_int, C.c_uint32,
                                  C.c_uint32, C.c_uint32]),
+    "glx_run_blocks_queued": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_int,
+                                        C.c_uint32]),
     "glx_init_accel": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _u32p, _dp, _dp, _dp, C.c_double]),
     "glx_kill_encounters": (C.c_int, [C.c_void_p]),
     "glx_partition": (C.c_int, [C.c_void_p, C.c_int, _u64p, _u64p]),
@@ -365,6 +367,13 @@ class Context:
         f = FORMS[form] if isinstance(form, str) else int(form)
         self._check(self.lib.glx_run_blocks(self.h, n_blocks, first_buf, dt, outer_radius, f, n_shards,
                                             waves_per_simd, launches_per_block))
+
+    def run_blocks_queued(self, n_blocks: int, first_buf: int, dt: float, outer_radius: float,
+                          form: str | int = "fp64", segment_steps: int = 0) -> None:
+        """run_blocks as one persistent launch taking (tile, segment) tasks
+        from a device queue (segment_steps: a multiple of 64, 0 = 256)."""
+        f = FORMS[form] if isinstance(form, str) else int(form)
+        self._check(self.lib.glx_run_blocks_queued(self.h, n_blocks, first_buf, dt, outer_radius, f, segment_steps))
 
     def init_accel(self, m, rh, planet_r, h0, outer_radius: float, ids=None) -> None:
         m = np.ascontiguousarray(m, dtype=np.float64)

@@ -164,6 +164,23 @@ int glx_run_block(glx_ctx* ctx, int buf, double dt, double outer_radius, int for
 int glx_run_blocks(glx_ctx* ctx, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
                    uint32_t n_shards, uint32_t waves_per_simd, uint32_t launches_per_block);
 
+/* The same integration as glx_run_blocks (same buffers, same results bit
+ * for bit, same deathtime_index convention) as ONE persistent launch: the
+ * particles are cut into tiles of 256 and every block into segments of
+ * segment_steps steps (a multiple of 64 up to 4096; 0 means 256).  The
+ * workgroups (at most one per tile) take tasks from a device FIFO that
+ * starts with every tile's first segment; finishing a tile's segment
+ * appends its next one.  A workgroup starts its next task as soon as it
+ * finishes one, so the SIMDs keep their resident waves until the last
+ * segments, where a chain of launches idles part of the device at the end
+ * of each one.  Async on the main stream (waits for the blocks' uploads).
+ * A wait for a FIFO entry is bounded in real time; should it ever expire
+ * (it cannot, short of a device fault) the next glx_sync or glx_partition
+ * returns GLX_EHIP.  The blocks must hold the same number of bodies and the
+ * same central mass.  No reference counterpart (see glx_run_blocks). */
+int glx_run_blocks_queued(glx_ctx* ctx, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
+                          uint32_t segment_steps);
+
 /* Initial accelerations and flags of particles [0, n_alive) from planet row
  * 0 of `planet_r` ((n_bodies-1) x 3) and h0 (3 doubles), async.  The
  * device-side counterpart of the helio_acc_particles<true> call in the

@@ -32,7 +32,7 @@ def test_profiled_counters_match_the_kernel_sources(bench):
     profile of the kernel sources whose SHA-256 bench.py records; the
     sources must still be those, or the profile is stale."""
     assert bench.PROFILED_KERNEL_SHA256 == bench.kernel_fingerprint(), (
-        "kernel sources changed since profiles/wh_block_200k_counters.md: re-profile and update bench.py")
+        "kernel sources changed since profiles/wh_queue_200k_counters.md: re-profile and update bench.py")
     for table in (bench.COUNTER_FLOPS_PER_PSTEP, bench.VALU_CYCLES_PER_PSTEP, bench.CLOCK_GHZ_MEASURED):
         assert set(table) == {"fp64", "fast", "mixed"}
     assert all(2.0 < c <= bench.PEAK_CLOCK_GHZ for c in bench.CLOCK_GHZ_MEASURED.values())

@@ -12,7 +12,8 @@ a CPU run of all 200,000 particles:
   * the device step count equals 200,000 x 4096 (no particle of this swarm is
     flagged in 2 blocks);
   * a 256-particle sample equals the oracle's particle path on the same
-    (CPU-interpolated) planet rows to 1e-12."""
+    (CPU-interpolated) planet rows to 1e-12;
+  * the queued persistent launch gives the same bits as the sliced one."""
 import numpy as np
 import pytest
 
@@ -37,7 +38,7 @@ def swarm():
     return s, h, np.full(s.n_bodies, CULL)
 
 
-def run(s, h, rh, form, n, shards, cap, parts):
+def run(s, h, rh, form, n, shards, cap, parts, queued=False):
     from glisser_amd import native
     with native.Context(n, s.n_bodies, TB, n_planet_buffers=BLOCKS) as ctx:
         ctx.upload(s.pr[:n], s.pv[:n], np.zeros(n, np.uint16), s.ids[:n], a=np.zeros((n, 3)),
@@ -51,7 +52,10 @@ def run(s, h, rh, form, n, shards, cap, parts):
         n_alive, _ = ctx.counts()
         for k in range(BLOCKS):
             ctx.interp_block(k, k * TB * DT, DT, TB, rh)
-        ctx.run_blocks(BLOCKS, 0, DT, OUTER, form, shards, cap, parts)
+        if queued:
+            ctx.run_blocks_queued(BLOCKS, 0, DT, OUTER, form, 0)
+        else:
+            ctx.run_blocks(BLOCKS, 0, DT, OUTER, form, shards, cap, parts)
         ctx.kill_encounters()
         steps = ctx.sum_steps(n_alive)
         ctx.partition()
@@ -74,6 +78,18 @@ def test_fullsize_equals_small_run(swarm, full, form):
     big, _ = full[form]
     for key in ("r", "v", "a", "flags", "id"):
         np.testing.assert_array_equal(big[key][:1000], small[key], err_msg=key)
+
+
+@pytest.mark.parametrize("form", ["fp64", "fast", "mixed"])
+def test_fullsize_queued_equals_sliced(swarm, full, form):
+    """The queued persistent launch (782 tiles x 16 segments of 256 steps)
+    gives every particle the bits of the sliced launch."""
+    s, h, rh = swarm
+    got, steps = run(s, h, rh, form, N, 0, 0, 0, queued=True)
+    ref, ref_steps = full[form]
+    assert steps == ref_steps
+    for key in ("r", "v", "a", "flags", "id"):
+        np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
 
 
 def test_fullsize_forms_agree(swarm, full):
