@@ -138,6 +138,7 @@ struct glx_ctx {
     unsigned char* q_pin = nullptr;
     hipEvent_t q_staged = nullptr;  // the staging's last copy has completed
     bool q_pending = false;         // a queued launch's error word is bound for pin_small[kQueueErrWord]
+    uint32_t n_cus = 0;             // compute units of the device
     std::vector<void*> allocs;
 };
 
@@ -324,6 +325,12 @@ int glx_create(const glx_params* p, glx_ctx** out) {
         return bail(rc);
     if (hipHostMalloc((void**)&c->pin_small, 64, hipHostMallocDefault) != hipSuccess)
         return bail(fail(GLX_ENOMEM, "hipHostMalloc failed"));
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+            return bail(fail(GLX_EHIP, "hipDeviceGetAttribute(multiprocessor count) failed"));
+        c->n_cus = (uint32_t)cus;
+    }
     const size_t log_doubles = (size_t)c->tb_max * c->nb_max * 3;
     for (auto& b : c->pb) {
         if ((rc = dalloc(c, &b.dev_log, log_doubles))) return bail(rc);
@@ -480,6 +487,9 @@ int check_buf(glx_ctx* c, int buf) {
     return GLX_OK;
 }
 
+int queue_run(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
+              uint32_t segment_steps);
+
 }  // namespace
 
 extern "C" {
@@ -490,6 +500,11 @@ int glx_run_block(glx_ctx* c, int buf, double dt, double outer_radius, int form)
     if (int rc = check_form(form)) return rc;
     if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
     GLX_HIP(hipSetDevice(c->device));
+    // With more 256-particle tiles than compute units, the launch's last
+    // workgroups would leave SIMDs on one wave at its end (the oldest wave
+    // issues first): the queued launch of this one block balances them and
+    // gives the same bits.
+    if ((c->n_alive + 255) / 256 > c->n_cus) return queue_run(c, 1, buf, dt, outer_radius, form, kQueueSegment);
     PlanetBuf& b = c->pb[buf];
     const BlockArgs A = block_args(c, b, dt, outer_radius, kLdsBudget);
     GLX_HIP(hipStreamWaitEvent(c->main_s, b.uploaded, 0));
@@ -653,28 +668,17 @@ int check_queue(glx_ctx* c) {
 
 extern "C" {
 
-int glx_run_blocks_queued(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
-                          uint32_t segment_steps) {
-    if (!c) return fail(GLX_EINVAL, "null context");
-    if (int rc = check_form(form)) return rc;
-    if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
+}  // extern "C"
+
+namespace {
+
+// One queued launch of n_blocks planet buffers from first_buf (arguments
+// checked by the caller: same bodies and central mass, segment_steps a
+// multiple of kRadiusPeriod).
+int queue_run(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
+              uint32_t segment_steps) {
     const int nbuf = (int)c->pb.size();
-    if (n_blocks < 1 || (int)n_blocks > nbuf)
-        return fail(GLX_EINVAL, "n_blocks %u outside [1, %d planet buffers]", n_blocks, nbuf);
-    if (segment_steps == 0) segment_steps = kQueueSegment;
-    if (segment_steps % glx::kRadiusPeriod != 0 || segment_steps > kQueueSegmentMax)
-        return fail(GLX_EINVAL, "segment_steps %u is not a multiple of %u up to %u", segment_steps,
-                    glx::kRadiusPeriod, kQueueSegmentMax);
-    if (first_buf < 0) return fail(GLX_EINVAL, "first_buf %d is negative", first_buf);
     const PlanetBuf& b0 = c->pb[first_buf % nbuf];
-    for (uint32_t k = 0; k < n_blocks; k++) {
-        const int buf = (first_buf + (int)k) % nbuf;
-        if (int rc = check_buf(c, buf)) return rc;
-        if (c->pb[buf].n_bodies != b0.n_bodies)
-            return fail(GLX_EINVAL, "planet buffers of one launch must hold the same bodies");
-        if (c->pb[buf].mu != b0.mu)
-            return fail(GLX_EINVAL, "planet buffers of one launch must hold the same central mass");
-    }
     GLX_HIP(hipSetDevice(c->device));
     if (int rc = ensure_queue(c)) return rc;
     // The staging may still feed the previous launch's table copy.
@@ -722,6 +726,35 @@ int glx_run_blocks_queued(glx_ctx* c, uint32_t n_blocks, int first_buf, double d
     for (uint32_t k = 0; k < n_blocks; k++) GLX_HIP(hipEventRecord(c->pb[(first_buf + (int)k) % nbuf].consumed, c->main_s));
     c->kernel_timed = true;
     return GLX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glx_run_blocks_queued(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, double outer_radius, int form,
+                          uint32_t segment_steps) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (int rc = check_form(form)) return rc;
+    if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
+    const int nbuf = (int)c->pb.size();
+    if (n_blocks < 1 || (int)n_blocks > nbuf)
+        return fail(GLX_EINVAL, "n_blocks %u outside [1, %d planet buffers]", n_blocks, nbuf);
+    if (segment_steps == 0) segment_steps = kQueueSegment;
+    if (segment_steps % glx::kRadiusPeriod != 0 || segment_steps > kQueueSegmentMax)
+        return fail(GLX_EINVAL, "segment_steps %u is not a multiple of %u up to %u", segment_steps,
+                    glx::kRadiusPeriod, kQueueSegmentMax);
+    if (first_buf < 0) return fail(GLX_EINVAL, "first_buf %d is negative", first_buf);
+    const PlanetBuf& b0 = c->pb[first_buf % nbuf];
+    for (uint32_t k = 0; k < n_blocks; k++) {
+        const int buf = (first_buf + (int)k) % nbuf;
+        if (int rc = check_buf(c, buf)) return rc;
+        if (c->pb[buf].n_bodies != b0.n_bodies)
+            return fail(GLX_EINVAL, "planet buffers of one launch must hold the same bodies");
+        if (c->pb[buf].mu != b0.mu)
+            return fail(GLX_EINVAL, "planet buffers of one launch must hold the same central mass");
+    }
+    return queue_run(c, n_blocks, first_buf, dt, outer_radius, form, segment_steps);
 }
 
 int glx_upload_history(glx_ctx* c, uint32_t n_bodies, const double* m, const uint32_t* id, uint64_t n_samples,

@@ -141,7 +141,10 @@ int glx_download_planet_block(glx_ctx* ctx, int buf, uint32_t n_bodies, uint32_t
                               double* h0_log);
 
 /* Advance particles [0, n_alive) through every step of planet buffer buf
- * (async on the main stream; waits for that buffer's upload).
+ * (async on the main stream; waits for that buffer's upload).  With more
+ * 256-particle tiles than the device has compute units it runs as the
+ * queued launch of glx_run_blocks_queued (one block, 256-step segments;
+ * same results), otherwise as one launch.
  * Replaces WHCudaIntegrator::integrate_particles_timeblock_cuda
  * (src/wh.cu:344-395) and the MVSKernel it launches (src/wh.cu:15-207). */
 int glx_run_block(glx_ctx* ctx, int buf, double dt, double outer_radius, int form);

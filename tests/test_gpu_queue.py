@@ -104,3 +104,15 @@ def test_queued_argument_checks():
             ctx.run_blocks_queued(1, 0, DT, OUTER, "fast", 64)
         with pytest.raises(native.GlxError, match="n_blocks"):
             ctx.run_blocks_queued(3, 0, DT, OUTER, "fast", 64)
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_run_block_above_the_cu_count_is_queued_and_equal(form):
+    """glx_run_block with more 256-particle tiles than compute units runs as a
+    queued launch of its one block: same bits as the plain single launch
+    (glx_run_blocks with one slice and one launch per block)."""
+    s, rh = _setup("three_body", 100_000)  # 391 tiles
+    ref = _run(s, rh, [2048], form, _sliced(form))
+    got = _run(s, rh, [2048], form, lambda ctx, nb: ctx.run_block(0, DT, OUTER, form))
+    assert int(np.count_nonzero(ref["flags"])) > 1000
+    _assert_same(got, ref)
