@@ -697,6 +697,10 @@ int queue_run(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, double ou
             segs[n_segs++] = glx::QueueSeg{k, r0, std::min(segment_steps, b.n_steps - r0), 0};
         offset += b.n_steps;
     }
+    const uint32_t n_tiles = (uint32_t)((c->n_alive + 255) / 256);
+    if ((uint64_t)n_segs * n_tiles + n_tiles >= 0xFFFFFFFFull)
+        return fail(GLX_EINVAL, "queued launch of %u segments x %u tiles: too many tasks for one launch", n_segs,
+                    n_tiles);
     glx::QueueArgs Q{};
     Q.base = block_args(c, b0, dt, outer_radius, kLdsBudget);
     Q.base.chunk = std::min<uint32_t>(Q.base.chunk, segment_steps);
@@ -705,7 +709,7 @@ int queue_run(glx_ctx* c, uint32_t n_blocks, int first_buf, double dt, double ou
     Q.ctl = c->q_ctl;
     Q.ring = c->q_ring;
     Q.n_segs = n_segs;
-    Q.n_tiles = (uint32_t)((c->n_alive + 255) / 256);
+    Q.n_tiles = n_tiles;
     Q.wait_ticks = kQueueWaitTicks;
     const size_t seg_bytes = (size_t)n_segs * sizeof(glx::QueueSeg);
     const size_t blk_bytes = (size_t)n_blocks * sizeof(glx::QueueBlock);
