@@ -136,7 +136,7 @@ def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int) -> int:
 # kernel is the profiled one.
 KERNEL_SOURCES = ["glisser_amd/csrc/wh_kernels.hip", "glisser_amd/csrc/wh_fast.h", "glisser_amd/csrc/wh_device.h",
                   "glisser_amd/csrc/sincos_cr.h", "glisser_amd/csrc/launch.h"]
-PROFILED_KERNEL_SHA256 = "66cec6244dc5a6ceafed29669800dcf438d9ce82a978a09ec9b23eeb98bd97eb"
+PROFILED_KERNEL_SHA256 = "7e70408b744c141ff93b5bf9850ec88e7fbd9dbbde29b9e5c7f3d1530e84ca30"
 
 
 def kernel_fingerprint() -> str:

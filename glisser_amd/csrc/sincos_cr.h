@@ -266,7 +266,8 @@ __host__ __device__ inline bool sincos_quick_tier(double x, double& s, double& c
     const double clo = ch.lo + (uh.lo + hl);
     s = sh.hi;
     c = ch.hi;
-    return rounds_to_hi(sh.hi, slo, zs) & rounds_to_hi(ch.hi, clo, zc);
+    // both tests evaluated, no branch between them
+    return (static_cast<unsigned>(rounds_to_hi(sh.hi, slo, zs)) & static_cast<unsigned>(rounds_to_hi(ch.hi, clo, zc))) != 0u;
 }
 
 // The quick phase for 2^-27 <= |x| <= 2^-2 (the tiers above).
