@@ -136,7 +136,7 @@ def hbm_bytes_per_block(n_particles: int, n_bodies: int, tbsize: int) -> int:
 # kernel is the profiled one.
 KERNEL_SOURCES = ["glisser_amd/csrc/wh_kernels.hip", "glisser_amd/csrc/wh_fast.h", "glisser_amd/csrc/wh_device.h",
                   "glisser_amd/csrc/sincos_cr.h", "glisser_amd/csrc/launch.h"]
-PROFILED_KERNEL_SHA256 = "7e70408b744c141ff93b5bf9850ec88e7fbd9dbbde29b9e5c7f3d1530e84ca30"
+PROFILED_KERNEL_SHA256 = "3f003a1ac264c57c115d5513e46aa8ca5f35ab7f569b8eacb3a7118f07685d5f"
 
 
 def kernel_fingerprint() -> str:
@@ -199,8 +199,7 @@ def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_s
         ctx = prepare_gpu(inputs, steps, warmup)
     try:
         if warmup:
-            for k in range(warmup):
-                ctx.interp_block(k, k * TBSIZE * DT, DT, TBSIZE, rh)
+            ctx.interp_blocks(0, [k * TBSIZE * DT for k in range(warmup)], DT, TBSIZE, rh)
             if launch == "queued":
                 ctx.run_blocks_queued(warmup, 0, DT, OUTER, form, segment_steps)
             else:
@@ -210,8 +209,7 @@ def run_gpu(inputs, form: str, steps: int, warmup: int, shards: int, waves_per_s
         ctx.sync()
         n_alive, _ = ctx.counts()
         t0 = time.perf_counter()
-        for k in range(warmup, warmup + steps):
-            ctx.interp_block(k, k * TBSIZE * DT, DT, TBSIZE, rh)
+        ctx.interp_blocks(warmup, [k * TBSIZE * DT for k in range(warmup, warmup + steps)], DT, TBSIZE, rh)
         if launch == "queued":
             ctx.run_blocks_queued(steps, warmup, DT, OUTER, form, segment_steps)
         else:

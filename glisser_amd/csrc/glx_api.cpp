@@ -859,6 +859,48 @@ int glx_interp_block(glx_ctx* c, int buf, double t_start, double dt, uint32_t n_
     return GLX_OK;
 }
 
+int glx_interp_blocks(glx_ctx* c, int first_buf, uint32_t n_blocks, const double* t_start, double dt, uint32_t n_steps,
+                      const double* rh) {
+    if (!c) return fail(GLX_EINVAL, "null context");
+    if (!c->hist.valid) return fail(GLX_ESTATE, "no planet history uploaded");
+    const int nbuf = (int)c->pb.size();
+    if (first_buf < 0) return fail(GLX_EINVAL, "first_buf %d is negative", first_buf);
+    if (n_blocks < 1 || (int)n_blocks > nbuf)
+        return fail(GLX_EINVAL, "n_blocks %u outside [1, %d planet buffers]", n_blocks, nbuf);
+    if (n_steps < 1 || n_steps > c->tb_max) return fail(GLX_EINVAL, "n_steps %u outside [1, %u]", n_steps, c->tb_max);
+    if (!(dt > 0.0)) return fail(GLX_EINVAL, "dt must be positive");
+    if (!rh || !t_start) return fail(GLX_EINVAL, "null rh or t_start");
+    for (uint32_t k = 0; k < n_blocks; k++) {
+        const double first = t_start[k] + dt, last = t_start[k] + (double)n_steps * dt;
+        if (!(first >= c->hist_t0 - kHistSlack) || !(last <= c->hist_t1 + kHistSlack))
+            return fail(GLX_EINVAL, "block %u times [%.17g, %.17g] leave the history [%.17g, %.17g]", k, first, last,
+                        c->hist_t0, c->hist_t1);
+    }
+    GLX_HIP(hipSetDevice(c->device));
+    const glx::HistoryView H = history_view(c);
+    for (uint32_t k0 = 0; k0 < n_blocks; k0 += glx::kRowBlocksMax) {
+        const uint32_t nk = std::min<uint32_t>(glx::kRowBlocksMax, n_blocks - k0);
+        glx::RowBlocks R{};
+        for (uint32_t k = 0; k < nk; k++) {
+            PlanetBuf& b = c->pb[(first_buf + (int)(k0 + k)) % nbuf];
+            GLX_HIP(hipStreamWaitEvent(c->htd_s, b.consumed, 0));
+            R.t_start[k] = t_start[k0 + k];
+            R.out[k] = b.dev_log;
+        }
+        GLX_HIP(glx::launch_hermite_rows_multi(H, R, nk, dt, n_steps, c->htd_s));
+        for (uint32_t k = 0; k < nk; k++) {
+            PlanetBuf& b = c->pb[(first_buf + (int)(k0 + k)) % nbuf];
+            GLX_HIP(hipEventRecord(b.uploaded, c->htd_s));
+            fill_consts(c->hist.n_bodies, c->hist.m, c->hist.id, rh, &b.pc);
+            b.mu = c->hist.m[0];
+            b.n_bodies = c->hist.n_bodies;
+            b.n_steps = n_steps;
+            b.valid = true;
+        }
+    }
+    return GLX_OK;
+}
+
 int glx_interp_at(glx_ctx* c, uint32_t n_times, const double* times, double* r, double* h0) {
     if (!c) return fail(GLX_EINVAL, "null context");
     if (!c->hist.valid) return fail(GLX_ESTATE, "no planet history uploaded");

@@ -35,7 +35,25 @@ __global__ __launch_bounds__(64) void hermite_rows_kernel(HistoryView H, const d
     hermite_row(H.t, H.r, H.v, H.n_samples, H.n_bodies, H.m, t, out + (size_t)s * H.n_bodies * 3);
 }
 
+// Several blocks' rows in one launch: block y of the grid writes n_steps
+// rows at R.t_start[y] + (s+1) dt into R.out[y].
+__global__ __launch_bounds__(64) void hermite_rows_multi_kernel(HistoryView H, RowBlocks R, double dt,
+                                                                uint32_t n_steps) {
+    const uint32_t s = blockIdx.x * 64 + threadIdx.x;
+    if (s >= n_steps) return;
+    const double t = step_time(R.t_start[blockIdx.y], s, dt);
+    hermite_row(H.t, H.r, H.v, H.n_samples, H.n_bodies, H.m, t, R.out[blockIdx.y] + (size_t)s * H.n_bodies * 3);
+}
+
 }  // namespace
+
+hipError_t launch_hermite_rows_multi(const HistoryView& H, const RowBlocks& R, uint32_t n_blocks, double dt,
+                                     uint32_t n_steps, hipStream_t stream) {
+    if (n_steps == 0 || n_blocks == 0) return hipSuccess;
+    if (n_blocks > kRowBlocksMax) return hipErrorInvalidValue;
+    hermite_rows_multi_kernel<<<dim3((n_steps + 63) / 64, n_blocks), dim3(64), 0, stream>>>(H, R, dt, n_steps);
+    return hipGetLastError();
+}
 
 hipError_t launch_hermite_rows(const HistoryView& H, const double* times, double t_start, double dt, uint32_t n_steps,
                                double* out, hipStream_t stream) {

@@ -73,6 +73,17 @@ struct HistoryView {
     double m[GLX_MAX_BODIES];
 };
 
+// Start times and output buffers of up to kRowBlocksMax blocks of rows.
+constexpr uint32_t kRowBlocksMax = 32;
+struct RowBlocks {
+    double t_start[kRowBlocksMax];
+    double* out[kRowBlocksMax];
+};
+// The rows of launch_hermite_rows (times null) for n_blocks blocks at once:
+// block k's n_steps rows at R.t_start[k] + (s+1) dt into R.out[k].
+hipError_t launch_hermite_rows_multi(const HistoryView& H, const RowBlocks& R, uint32_t n_blocks, double dt,
+                                     uint32_t n_steps, hipStream_t stream);
+
 // Planet-log rows (h0, then planet positions; n_bodies x 3 doubles each) at
 // times[s], or at t_start + (s+1) dt when times is null, for s < n_steps.
 hipError_t launch_hermite_rows(const HistoryView& H, const double* times, double t_start, double dt, uint32_t n_steps,

@@ -145,6 +145,7 @@ SIGNATURES = {
     "glx_upload_planet_block": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, _dp, _u32p, _dp, _dp, _dp, C.c_uint32]),
     "glx_upload_history": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _u32p, C.c_uint64, _dp, _dp, _dp]),
     "glx_interp_block": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_uint32, _dp]),
+    "glx_interp_blocks": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, _dp, C.c_double, C.c_uint32, _dp]),
     "glx_interp_at": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _dp, _dp]),
     "glx_download_planet_block": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, _dp, _dp]),
     "glx_run_block": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int]),
@@ -342,6 +343,13 @@ class Context:
     def interp_block(self, buf: int, t_start: float, dt: float, n_steps: int, rh) -> None:
         rh = np.ascontiguousarray(rh, dtype=np.float64)
         self._check(self.lib.glx_interp_block(self.h, buf, t_start, dt, n_steps, _ptr(rh, _dp)))
+
+    def interp_blocks(self, first_buf: int, t_start, dt: float, n_steps: int, rh) -> None:
+        """interp_block for len(t_start) consecutive buffers in one launch."""
+        rh = np.ascontiguousarray(rh, dtype=np.float64)
+        t_start = np.ascontiguousarray(np.atleast_1d(t_start), dtype=np.float64)
+        self._check(self.lib.glx_interp_blocks(self.h, first_buf, len(t_start), _ptr(t_start, _dp), dt, n_steps,
+                                               _ptr(rh, _dp)))
 
     def interp_at(self, times):
         """(r (n, n_bodies-1, 3), h0 (n, 3)) interpolated at the given times."""

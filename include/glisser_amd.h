@@ -129,6 +129,14 @@ int glx_upload_history(glx_ctx* ctx, uint32_t n_bodies, const double* m, const u
  * src/executor.cu:442-461). */
 int glx_interp_block(glx_ctx* ctx, int buf, double t_start, double dt, uint32_t n_steps, const double* rh);
 
+/* glx_interp_block for n_blocks buffers first_buf, first_buf+1, ... (modulo
+ * the buffer count), block k starting at t_start[k], in one launch per 32
+ * blocks: the same rows, without a host round of API calls per block ahead
+ * of a multi-block run.  No reference counterpart (the reference fills one
+ * block per host iteration). */
+int glx_interp_blocks(glx_ctx* ctx, int first_buf, uint32_t n_blocks, const double* t_start, double dt,
+                      uint32_t n_steps, const double* rh);
+
 /* Interpolated planet positions (n_times x (n_bodies-1) x 3) and h0
  * (n_times x 3) at arbitrary times, synchronous; either output may be NULL.
  * Interpolator::fill_one (src/interp.cpp:99-111) plus load_h0. */

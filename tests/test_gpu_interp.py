@@ -98,3 +98,29 @@ def test_particles_from_device_history(native, oracle, hist_system):
     np.testing.assert_array_equal(gpu["flags"], cpu["flags"])
     assert rel_err(gpu["r"], cpu["r"]).max() < 1e-12
     assert rel_err(gpu["v"], cpu["v"]).max() < 1e-12
+
+
+def test_interp_blocks_equals_per_block_calls():
+    """glx_interp_blocks (one launch for several blocks, 40 here: two launches
+    of at most 32) fills every buffer with the rows n glx_interp_block calls
+    write: bit for bit, and the particle kernel runs on them alike."""
+    from glisser_amd import native, systems
+    s = systems.outer_solar_system(8, seed=5)
+    nblk, tb, dt = 40, 96, 180.0
+    h = systems.planet_history(s, (nblk + 1) * tb * dt + 365.0, spacing=365.0, substeps=10)
+    rh = np.full(s.n_bodies, 0.5)
+    starts = [k * tb * dt for k in range(nblk)]
+    rows = []
+    for multi in (False, True):
+        with native.Context(8, s.n_bodies, tb, n_planet_buffers=nblk) as ctx:
+            ctx.upload_history(h.m, h.t, h.r, h.v)
+            if multi:
+                ctx.interp_blocks(0, starts, dt, tb, rh)
+            else:
+                for k, t0 in enumerate(starts):
+                    ctx.interp_block(k, t0, dt, tb, rh)
+            ctx.sync()
+            rows.append([ctx.download_planet_block(k, s.n_bodies, tb) for k in range(nblk)])
+    for (r0, h0), (r1, h1) in zip(*rows):
+        np.testing.assert_array_equal(r0, r1)
+        np.testing.assert_array_equal(h0, h1)
