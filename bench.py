@@ -537,8 +537,8 @@ def cpu_track(inputs, records) -> dict:
     """The reference's own save_binary_track (oracle/_ref) over the same
     particles, already in id order (its sort_by_id is not timed), to a track
     file: host clock, median of 3; and, as the checker, the device records
-    against the records of that file (same ids; Cartesian-to-element floats
-    equal or one float32 ulp apart)."""
+    against the records of that file (same ids; the share of element floats
+    equal bit for bit, and the largest float32 ulp distance)."""
     import tempfile
     from oracle.pyoracle import Oracle
 

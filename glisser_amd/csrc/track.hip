@@ -11,6 +11,12 @@
 // below 4x the particle capacity), so no sort runs at all.
 #include <hip/hip_runtime.h>
 
+#include <string>
+#include <vector>
+
+#include "elem_cr.h"
+#include "errors.h"
+#include "glisser_amd.h"
 #include "launch.h"
 
 namespace glx {
@@ -27,10 +33,14 @@ uint32_t grid_for(uint64_t n) {
 
 // Osculating elements about central GM mu, in the operation order of the
 // host conversion (src/convert.cpp:310-463); written to a[6] as
-// a, e, i, Omega, omega, f.
-__device__ void elements_of(double mu, double x, double y, double z, double vx, double vy, double vz, double* out) {
+// a, e, i, Omega, omega, f.  Its atan2, acos, asin, sin and cos are the
+// correctly rounded ones (elem_cr.h, sincos_cr.h), which glibc's results
+// equal in all but rare cases; pi is 2 asin(1) rounded, as the host has it.
+// The same code runs on the host (glx_track_elements_host).
+__host__ __device__ void elements_of(double mu, double x, double y, double z, double vx, double vy, double vz,
+                                     double* out) {
 #pragma clang fp contract(off)
-    const double pi = 2.0 * asin(1.0);
+    constexpr double pi = 0x1.921fb54442d18p+1;
     const double prec = 1.0e-13;
     const double hx = y * vz - z * vy;
     const double hy = z * vx - x * vz;
@@ -39,19 +49,20 @@ __device__ void elements_of(double mu, double x, double y, double z, double vx, 
     if (!(hsq > prec)) {  // radial orbit
         out[0] = sqrt(x * x + y * y + z * z);
         out[1] = HUGE_VAL;
-        out[2] = asin(z / sqrt(x * x + y * y + z * z));
-        out[3] = atan2(y, x);
+        out[2] = cr::asin_cr(z / sqrt(x * x + y * y + z * z));
+        out[3] = cr::atan2_cr(y, x);
         out[4] = HUGE_VAL;
         out[5] = HUGE_VAL;
         return;
     }
-    const double i = acos(hz / sqrt(hsq));
-    const double capom = (fabs(i) < prec || fabs(pi - fabs(i)) < prec) ? 0.0 : atan2(hx, -hy);
+    const double i = cr::acos_cr(hz / sqrt(hsq));
+    const double capom = (fabs(i) < prec || fabs(pi - fabs(i)) < prec) ? 0.0 : cr::atan2_cr(hx, -hy);
     const double vsq = vx * vx + vy * vy + vz * vz;
     const double vdotr = x * vx + y * vy + z * vz;
     const double rr = sqrt(x * x + y * y + z * z);
     const double xhat = x / rr, yhat = y / rr, zhat = z / rr;
-    const double nx = cos(capom), ny = sin(capom);
+    double nx, ny;
+    cr::sincos_cr(capom, &ny, &nx);
     double fac = vsq * rr - mu;
     const double Px = fac * xhat - vdotr * vx;
     const double Py = fac * yhat - vdotr * vy;
@@ -62,24 +73,24 @@ __device__ void elements_of(double mu, double x, double y, double z, double vx, 
     if (fabs(e) < prec) {
         om = 0.0;
     } else if ((i < prec) || (pi - i < prec)) {
-        om = atan2(Py, Px);
+        om = cr::atan2_cr(Py, Px);
     } else {
         const double ecosw = (nx * Px + ny * Py) / mu;
-        om = acos(ecosw / e);
+        om = cr::acos_cr(ecosw / e);
         if (fabs(Pz) > prec) om *= fabs(Pz) / Pz;
     }
     const double energy = vsq / 2.0 - mu / rr;
     double a, f;
     if (fabs(energy) < prec) {
         a = 0.5 * hsq / mu;
-        f = fabs(vdotr) < prec ? 0.0 : 2.0 * acos(sqrt(a / rr)) * vdotr / fabs(vdotr);
+        f = fabs(vdotr) < prec ? 0.0 : 2.0 * cr::acos_cr(sqrt(a / rr)) * vdotr / fabs(vdotr);
     } else if (energy > 0.0) {
         a = -0.5 * mu / energy;
         if (fabs(vdotr) < prec) {
             f = 0.0;
         } else {
             fac = a * (1.0 - e * e) / rr - 1.0;
-            f = acos(fac / e) * vdotr / fabs(vdotr);
+            f = cr::acos_cr(fac / e) * vdotr / fabs(vdotr);
         }
     } else {
         a = -0.5 * mu / energy;
@@ -88,16 +99,18 @@ __device__ void elements_of(double mu, double x, double y, double z, double vx, 
                 f = rr < a ? 0.0 : pi;
             } else {
                 fac = a * (1.0 - e * e) / rr - 1.0;
-                f = acos(fac / e) * vdotr / fabs(vdotr);
-                if (isnan(f)) f = rr < a ? 0.0 : pi;
+                f = cr::acos_cr(fac / e) * vdotr / fabs(vdotr);
+                if (f != f) f = rr < a ? 0.0 : pi;  // NaN
             }
         } else {
             fac = (x * nx + y * ny) / rr;
-            f = acos(fac);
+            f = cr::acos_cr(fac);
             if (fabs(z) > prec) {
                 f *= fabs(z) / z;
             } else if ((i < prec) || (pi - i < prec)) {
-                f = atan2(y, x) * cos(i);
+                double si, ci;
+                cr::sincos_cr(i, &si, &ci);
+                f = cr::atan2_cr(y, x) * ci;
             }
         }
     }
@@ -230,6 +243,26 @@ __global__ __launch_bounds__(kThreads) void pack_track_kernel(ParticleSet s, uin
     }
 }
 
+// Elements of n Cartesian rows (x, y, z, vx, vy, vz) into n rows of six.
+__global__ __launch_bounds__(kThreads) void elements_kernel(double mu, const double* __restrict__ rv, double* el,
+                                                            uint64_t n) {
+    const uint64_t k = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (k >= n) return;
+    const double* q = rv + 6 * k;
+    elements_of(mu, q[0], q[1], q[2], q[3], q[4], q[5], el + 6 * k);
+}
+
+__host__ __device__ inline double cr_math(int fn, double a, double b) {
+    return fn == GLX_CR_ATAN2 ? cr::atan2_cr(a, b) : fn == GLX_CR_ACOS ? cr::acos_cr(a) : cr::asin_cr(a);
+}
+
+__global__ __launch_bounds__(kThreads) void cr_math_kernel(int fn, const double* __restrict__ a,
+                                                           const double* __restrict__ b, double* out, uint64_t n) {
+    const uint64_t k = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (k >= n) return;
+    out[k] = cr_math(fn, a[k], fn == GLX_CR_ATAN2 ? b[k] : 0.0);
+}
+
 // max over id[0, n) folded into *out (cleared beforehand): each workgroup
 // reduces its grid-stride share, then one atomic max per workgroup.
 __global__ __launch_bounds__(kThreads) void max_id_kernel(const uint32_t* __restrict__ id, uint32_t n, uint32_t* out) {
@@ -277,3 +310,85 @@ hipError_t launch_pack_track(ParticleSet cur, uint32_t n, const uint32_t* rank, 
 }
 
 }  // namespace glx
+
+namespace {
+
+int hip_fail(const char* what, hipError_t e) {
+    return glx::set_error(GLX_EHIP, std::string(what) + " failed: " + hipGetErrorString(e));
+}
+
+// Runs `launch(device_in, device_out)` on GPU `device` over n_in input and
+// n_out output doubles copied in and out synchronously.
+template <class F>
+int run_on_device(int device, const double* in, uint64_t n_in, double* out, uint64_t n_out, F launch) {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail("hipSetDevice", e);
+    double* d = nullptr;
+    e = hipMalloc((void**)&d, (n_in + n_out) * sizeof(double));
+    if (e != hipSuccess) return glx::set_error(GLX_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    int rc = GLX_OK;
+    if ((e = hipMemcpy(d, in, n_in * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) {
+        rc = hip_fail("hipMemcpy", e);
+    } else {
+        launch(d, d + n_in);
+        if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess)
+            rc = hip_fail("kernel", e);
+        else if ((e = hipMemcpy(out, d + n_in, n_out * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            rc = hip_fail("hipMemcpy", e);
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
+constexpr uint64_t kMaxRows = (uint64_t)1 << 26;
+
+int check_cr_args(int fn, uint64_t n, const double* a, const double* b, const double* out) {
+    if (fn != GLX_CR_ATAN2 && fn != GLX_CR_ACOS && fn != GLX_CR_ASIN)
+        return glx::set_error(GLX_EINVAL, "fn must be GLX_CR_ATAN2, GLX_CR_ACOS or GLX_CR_ASIN");
+    if (n > 0 && (!a || !out || (fn == GLX_CR_ATAN2 && !b))) return glx::set_error(GLX_EINVAL, "null argument");
+    return GLX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glx_track_elements_host(uint64_t n, double mu, const double* rv, double* el) {
+    if (n > 0 && (!rv || !el)) return glx::set_error(GLX_EINVAL, "null argument");
+    for (uint64_t k = 0; k < n; k++)
+        glx::elements_of(mu, rv[6 * k], rv[6 * k + 1], rv[6 * k + 2], rv[6 * k + 3], rv[6 * k + 4], rv[6 * k + 5],
+                         el + 6 * k);
+    return GLX_OK;
+}
+
+int glx_track_elements_device(int device, uint64_t n, double mu, const double* rv, double* el) {
+    if (n > 0 && (!rv || !el)) return glx::set_error(GLX_EINVAL, "null argument");
+    if (n == 0) return GLX_OK;
+    if (n > kMaxRows) return glx::set_error(GLX_EINVAL, "at most 2^26 states per call");
+    return run_on_device(device, rv, 6 * n, el, 6 * n, [&](const double* din, double* dout) {
+        hipLaunchKernelGGL(glx::elements_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, mu, din, dout,
+                           n);
+    });
+}
+
+int glx_cr_math_host(int fn, uint64_t n, const double* a, const double* b, double* out) {
+    if (int rc = check_cr_args(fn, n, a, b, out)) return rc;
+    for (uint64_t k = 0; k < n; k++) out[k] = glx::cr_math(fn, a[k], fn == GLX_CR_ATAN2 ? b[k] : 0.0);
+    return GLX_OK;
+}
+
+int glx_cr_math_device(int device, int fn, uint64_t n, const double* a, const double* b, double* out) {
+    if (int rc = check_cr_args(fn, n, a, b, out)) return rc;
+    if (n == 0) return GLX_OK;
+    if (n > kMaxRows) return glx::set_error(GLX_EINVAL, "at most 2^26 values per call");
+    // one device buffer: a, then b for atan2
+    const bool two = fn == GLX_CR_ATAN2;
+    std::vector<double> host_in(a, a + n);
+    if (two) host_in.insert(host_in.end(), b, b + n);
+    return run_on_device(device, host_in.data(), host_in.size(), out, n, [&](const double* din, double* dout) {
+        hipLaunchKernelGGL(glx::cr_math_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, 0, fn, din,
+                           two ? din + n : din, dout, n);
+    });
+}
+
+}  // extern "C"

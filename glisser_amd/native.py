@@ -165,6 +165,10 @@ SIGNATURES = {
     "glx_partition_static": (C.c_int, [C.POINTER(DeviceParticles), C.POINTER(DeviceParticles), C.c_uint64, C.c_int,
                                        C.c_void_p, C.c_void_p]),
     "glx_sincos_host": (C.c_int, [C.c_uint64, _dp, _dp, _dp]),
+    "glx_track_elements_device": (C.c_int, [C.c_int, C.c_uint64, C.c_double, _dp, _dp]),
+    "glx_track_elements_host": (C.c_int, [C.c_uint64, C.c_double, _dp, _dp]),
+    "glx_cr_math_device": (C.c_int, [C.c_int, C.c_int, C.c_uint64, _dp, _dp, _dp]),
+    "glx_cr_math_host": (C.c_int, [C.c_int, C.c_uint64, _dp, _dp, _dp]),
     "glx_batch_create": (C.c_int, [C.c_int, C.POINTER(SystemDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "glx_batch_destroy": (C.c_int, [C.c_void_p]),
     "glx_batch_upload": (C.c_int, [C.c_void_p, C.c_uint32, _dp, _dp, _u16p, _u32p]),
@@ -898,6 +902,43 @@ def from_elements(mu, el, mean_anomaly: bool = True):
     _check_host(load().glx_from_elements(n, _ptr(mu, _dp), _ptr(el, _dp), int(mean_anomaly), _ptr(r, _dp),
                                          _ptr(v, _dp)))
     return r, v
+
+
+def track_elements(mu: float, rv, device: int | None = None):
+    """Elements (a, e, i, Omega, omega, f) of Cartesian rows rv (n x 6) about
+    central GM mu, as the device track output forms them: on GPU `device`,
+    or with the host twin of the same code when device is None."""
+    rv = np.ascontiguousarray(np.atleast_2d(rv), dtype=np.float64)
+    if rv.ndim != 2 or rv.shape[1] != 6:
+        raise ValueError("rv must have shape (n, 6)")
+    n = rv.shape[0]
+    el = np.empty((n, 6))
+    lib = load()
+    if device is None:
+        _check_host(lib.glx_track_elements_host(n, float(mu), _ptr(rv, _dp), _ptr(el, _dp)))
+    else:
+        _check_host(lib.glx_track_elements_device(int(device), n, float(mu), _ptr(rv, _dp), _ptr(el, _dp)))
+    return el
+
+
+CR_FN = {"atan2": 0, "acos": 1, "asin": 2}
+
+
+def cr_math(fn: str, a, b=None, device: int | None = None):
+    """The track conversion's correctly rounded atan2(a, b), acos(a) or
+    asin(a) over arrays: on GPU `device`, or on the host when device is None."""
+    a = np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)
+    n = a.shape[0]
+    if fn == "atan2":
+        b = np.ascontiguousarray(np.broadcast_to(np.asarray(b, dtype=np.float64), (n,)))
+    out = np.empty(n)
+    bp = _ptr(b, _dp) if b is not None else None
+    lib = load()
+    if device is None:
+        _check_host(lib.glx_cr_math_host(CR_FN[fn], n, _ptr(a, _dp), bp, _ptr(out, _dp)))
+    else:
+        _check_host(lib.glx_cr_math_device(int(device), CR_FN[fn], n, _ptr(a, _dp), bp, _ptr(out, _dp)))
+    return out
 
 
 def sincos(x, device: int | None = None):

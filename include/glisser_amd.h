@@ -509,6 +509,24 @@ int glx_track_pack(glx_ctx* ctx, uint32_t n_planets, const double* pm, const dou
 int glx_track_write_records(const char* path, int append, double time, const glx_state* s, int to_elements,
                             int barycentric, const void* records, uint64_t n);
 
+/* The element conversion glx_track_pack runs on the device, as doubles:
+ * n Cartesian rows (x, y, z, vx, vy, vz) about central GM mu into n rows
+ * (a, e, i, Omega, omega, f), in the operation order of the reference's
+ * to_elements (src/convert.cpp:310-463) with correctly rounded atan2, acos,
+ * asin, sin and cos, the values glibc returns in all but rare cases.
+ * glx_track_elements_device runs the device code on GPU `device`
+ * (synchronous; at most 2^26 rows); glx_track_elements_host runs the same
+ * code on the host. */
+int glx_track_elements_device(int device, uint64_t n, double mu, const double* rv, double* el);
+int glx_track_elements_host(uint64_t n, double mu, const double* rv, double* el);
+
+/* The correctly rounded functions of that conversion over arrays:
+ * out[k] = atan2(a[k], b[k]), acos(a[k]) or asin(a[k]) (b unused), on GPU
+ * `device` (synchronous; at most 2^26 values) or on the host. */
+enum glx_cr_fn { GLX_CR_ATAN2 = 0, GLX_CR_ACOS = 1, GLX_CR_ASIN = 2 };
+int glx_cr_math_device(int device, int fn, uint64_t n, const double* a, const double* b, double* out);
+int glx_cr_math_host(int fn, uint64_t n, const double* a, const double* b, double* out);
+
 /* ---------------------------------------------------------------------
  * Planet history files (README.md:154-171): the reference's "planetary
  * history" of float32 heliocentric elements per record, written by
