@@ -23,9 +23,10 @@
 // unless the exact value lies that close to a rounding boundary.
 // acos x = atan2(sqrt((1 - x)(1 + x)), x), asin x = atan2(x, sqrt((1 - x)(1 +
 // x))), with 1 - x and 1 + x exact pairs.  Zeros, infinities, NaN, |x| >= 1
-// and quotients below 2^-900 take the math library, whose results there are
-// exact special values (0, +-pi, +-pi/2, +-pi/4, +-3pi/4 rounded, NaN) or
-// below the smallest normal quotient's ulp.
+// and operands whose quotient is below 2^-900 take the math library: there
+// the result is a special value (+-0, +-pi, +-pi/2, +-pi/4, +-3pi/4 rounded,
+// NaN), or +-pi rounded, or the quotient min/max itself, from which atan
+// differs by a factor 1 - t^2/3 that is 1 to far below an ulp.
 // The constants are atan(k/32), (-1)^n / (2n + 1), pi/2 and pi as pairs
 // (rounded value, rounded remainder) from 300-bit arithmetic
 // (gen_elem_cr_table.py).  FMA contraction is off: every fused operation is
